@@ -1,0 +1,176 @@
+"""cusat_amd -- MI355X (gfx950) RX FEC hot path of cuSat.
+
+Python access to the C ABI of ``include/cusat_fec.h`` (libcusat_fec.so,
+built in-tree by ``cusat_amd/build.py``).  Device buffers are passed as raw
+pointers; torch tensors are accepted wherever a device pointer is expected
+(``tensor.data_ptr()``), and the stream defaults to torch's current stream.
+
+The library must be built; importing works without it, but every call
+raises if ``libcusat_fec.so`` is missing -- there is no fallback path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libcusat_fec.so")
+DATA_DIR = os.path.join(_PKG, "data")
+
+FECFRAME_NORMAL, FECFRAME_SHORT, FECFRAME_MEDIUM = 0, 1, 2
+MODULATIONS = ["QPSK", "8PSK", "8APSK", "16APSK", "32APSK", "64APSK", "128APSK", "256APSK"]
+RATES = ["1/4", "1/3", "2/5", "1/2", "3/5", "2/3", "3/4", "4/5", "5/6", "8/9", "9/10",
+         "11/45", "4/15", "13/45", "14/45", "9/20", "7/15", "90/180", "8/15", "96/180",
+         "11/20", "100/180", "26/45", "104/180", "18/30", "28/45", "23/36", "116/180",
+         "20/30", "124/180", "25/36", "32/45", "128/180", "13/18", "22/30", "132/180",
+         "135/180", "7/9", "140/180", "154/180"]
+CTX_SEQUENTIAL_SCHEDULE = 0x1
+
+
+def rate_id(r):
+    return r if isinstance(r, int) else RATES.index(r)
+
+
+def mod_id(m):
+    return m if isinstance(m, int) else MODULATIONS.index(m.upper())
+
+
+class FecInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in (
+        "n_ldpc", "k_ldpc", "k_bch", "n_bch", "bch_t", "bch_degree", "q", "n_steps", "lanes",
+        "cnl", "lt_alloc", "n_mod", "xfec_len", "demap_len", "max_batch", "max_trials",
+        "n_collision_slots", "lds_bytes")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """The loaded libcusat_fec.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libcusat_fec.so is not built: run python cusat_amd/build.py")
+        # torch ships its own libamdhip64.so.7; load it first so this library
+        # binds to the same HIP runtime instance (streams and device pointers
+        # from torch are then valid here).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i, f, u = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_uint
+        L.cusat_last_error.restype = ctypes.c_char_p
+        L.cusat_fec_ctx_create.argtypes = [ctypes.POINTER(vp), i, i, i, i, i, u]
+        L.cusat_fec_ctx_destroy.argtypes = [vp]
+        L.cusat_fec_ctx_info.argtypes = [vp, ctypes.POINTER(FecInfo)]
+        L.cusat_fec_query.argtypes = [i, i, i, u, ctypes.POINTER(FecInfo)]
+        L.cusat_fec_demap_scales.argtypes = [vp, f, f, ctypes.POINTER(f), ctypes.POINTER(f),
+                                             ctypes.POINTER(f)]
+        L.cusat_fec_constellation.argtypes = [vp, vp, vp]
+        L.cusat_fec_demap.argtypes = [vp, vp, vp, i, f, f, vp]
+        L.cusat_ldpc_decode.argtypes = [vp, vp, vp, vp, i, vp]
+        L.cusat_bch_check.argtypes = [vp, vp, vp, i, vp]
+        L.cusat_rx_fec_batch.argtypes = [vp, vp, vp, i, f, f, vp, vp, vp, vp]
+        L.cusat_gen_frames.argtypes = [i, i, i, ctypes.c_uint64, i, f, vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError("%s failed (%d): %s" % (what, rc, lib().cusat_last_error().decode()))
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    if hasattr(x, "data_ptr"):
+        return ctypes.c_void_p(x.data_ptr())
+    if isinstance(x, np.ndarray):
+        return x.ctypes.data_as(ctypes.c_void_p)
+    return ctypes.c_void_p(int(x))
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    if hasattr(stream, "cuda_stream"):
+        return ctypes.c_void_p(stream.cuda_stream)
+    return ctypes.c_void_p(int(stream))
+
+
+class FecContext:
+    """One MODCOD's decoder state on the current HIP device
+    (demapper_cuda + ldpc_decode_cuda + bch_check_cuda of the reference)."""
+
+    def __init__(self, framesize, rate, modulation, max_trials=50, max_batch=64, flags=0):
+        self.framesize, self.rate, self.modulation = framesize, rate_id(rate), mod_id(modulation)
+        h = ctypes.c_void_p()
+        _check(lib().cusat_fec_ctx_create(ctypes.byref(h), framesize, self.rate, self.modulation,
+                                          max_trials, max_batch, flags), "cusat_fec_ctx_create")
+        self._h = h
+        inf = FecInfo()
+        _check(lib().cusat_fec_ctx_info(h, ctypes.byref(inf)), "cusat_fec_ctx_info")
+        self.info = inf.as_dict()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cusat_fec_ctx_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def demap_scales(self, noise_var, llr_gain):
+        a, b, c = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+        _check(lib().cusat_fec_demap_scales(self._h, noise_var, llr_gain, ctypes.byref(a),
+                                            ctypes.byref(b), ctypes.byref(c)), "demap_scales")
+        return a.value, b.value, c.value
+
+    def constellation(self):
+        pts = np.zeros(2 << self.info["n_mod"], np.float32)
+        col = np.zeros(8, np.uint32)
+        _check(lib().cusat_fec_constellation(self._h, _ptr(pts), _ptr(col)), "constellation")
+        return pts, col
+
+    def demap(self, sym, llr, batch, noise_var, llr_gain, stream=None):
+        _check(lib().cusat_fec_demap(self._h, _ptr(sym), _ptr(llr), batch, noise_var, llr_gain,
+                                     _stream(stream)), "cusat_fec_demap")
+
+    def decode(self, llr, hard, trials, batch, stream=None):
+        _check(lib().cusat_ldpc_decode(self._h, _ptr(llr), _ptr(hard), _ptr(trials), batch,
+                                       _stream(stream)), "cusat_ldpc_decode")
+
+    def bch_check(self, hard, valid, batch, stream=None):
+        _check(lib().cusat_bch_check(self._h, _ptr(hard), _ptr(valid), batch, _stream(stream)),
+               "cusat_bch_check")
+
+    def rx_fec_batch(self, sym, llr, batch, noise_var, llr_gain, hard, trials, valid, stream=None):
+        _check(lib().cusat_rx_fec_batch(self._h, _ptr(sym), _ptr(llr), batch, noise_var, llr_gain,
+                                        _ptr(hard), _ptr(trials), _ptr(valid), _stream(stream)),
+               "cusat_rx_fec_batch")
+
+
+def gen_frames(framesize, rate, modulation, seed, batch, es_n0_db, want_cw=False):
+    """Seeded transmit chain on the host CPU.  Returns (symbols float32
+    [B, xfec, 2], info bytes uint8 [B, k_bch/8], codeword bytes or None)."""
+    rate, modulation = rate_id(rate), mod_id(modulation)
+    info = geometry(framesize, rate, modulation)
+    sym = np.zeros((batch, info["xfec_len"], 2), np.float32)
+    inf = np.zeros((batch, info["k_bch"] // 8), np.uint8)
+    cw = np.zeros((batch, info["n_ldpc"] // 8), np.uint8) if want_cw else None
+    _check(lib().cusat_gen_frames(framesize, rate, modulation, seed, batch, es_n0_db, _ptr(sym),
+                                  _ptr(inf), _ptr(cw)), "cusat_gen_frames")
+    return sym, inf, cw
+
+
+def geometry(framesize, rate, modulation, flags=0):
+    """Code/modulation dimensions from the host tables (no device needed)."""
+    inf = FecInfo()
+    _check(lib().cusat_fec_query(framesize, rate_id(rate), mod_id(modulation), flags,
+                                 ctypes.byref(inf)), "cusat_fec_query")
+    return inf.as_dict()

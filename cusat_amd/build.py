@@ -1,0 +1,85 @@
+#!/usr/bin/env python3
+"""Build libcusat_fec.so (gfx950 HIP kernels + C ABI + host code) in-tree.
+
+    python -m cusat_amd.build        # or: python cusat_amd/build.py
+
+Every call recompiles.  Objects go to cusat_amd/_build/, the library to
+cusat_amd/libcusat_fec.so.
+"""
+import concurrent.futures
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+BUILD = os.path.join(PKG, "_build")
+LIB = os.path.join(PKG, "libcusat_fec.so")
+ARCH = os.environ.get("CUSAT_OFFLOAD_ARCH", "gfx950")
+
+SOURCES = ["codes.cc", "modulation.cc", "bch.cc", "gen.cc", "capi.cc",
+           "ldpc_decode.hip", "demap.hip", "bch_check.hip"]
+
+
+def hipcc():
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def gen_data_header():
+    out_dir = os.path.join(CSRC, "_gen")
+    os.makedirs(out_dir, exist_ok=True)
+    parts = []
+    for sym, fname in (("k_ldpc_tables_txt", "ldpc_tables.txt"),
+                       ("k_ldpc_schedules_txt", "ldpc_schedules.txt")):
+        data = open(os.path.join(PKG, "data", fname), "rb").read()
+        body = ",".join(str(b) for b in data)
+        parts.append("static const char %s[] = {%s,0};\n" % (sym, body))
+    path = os.path.join(out_dir, "data_inc.h")
+    text = "// generated by cusat_amd/build.py from cusat_amd/data/*.txt\n#pragma once\n" + "".join(parts)
+    if not os.path.exists(path) or open(path).read() != text:
+        with open(path, "w") as f:
+            f.write(text)
+
+
+def compile_one(src):
+    cc = hipcc()
+    obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+    common = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
+              "-I", os.path.join(ROOT, "include"), "-I", CSRC]
+    if src.endswith(".hip"):
+        cmd = [cc, "-x", "hip", "--offload-arch=" + ARCH, "-ffp-contract=off",
+               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+               "-munsafe-fp-atomics"] + common + ["-c", os.path.join(CSRC, src), "-o", obj]
+    else:
+        cmd = [cc, "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__"] + common + \
+              ["-c", os.path.join(CSRC, src), "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("compile failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    return obj
+
+
+def build(verbose=True):
+    os.makedirs(BUILD, exist_ok=True)
+    gen_data_header()
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 4)) as ex:
+        objs = list(ex.map(compile_one, SOURCES))
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs + ["-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))
+    os.replace(tmp, LIB)
+    if verbose:
+        print("built", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

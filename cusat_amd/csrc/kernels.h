@@ -1,0 +1,52 @@
+// Launchers of the gfx950 kernels (host side declarations).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "codes.h"
+
+namespace cusat {
+
+struct ldpc_args {
+    const int16_t* llr;        // [B][N]
+    int8_t* msg;               // [B][lt_alloc] check-to-variable messages
+    uint8_t* hard;             // [B][K/8]
+    int32_t* result;           // [B]
+    const uint32_t* circ;      // packed circulants, per layer
+    const int* lay_cnt;        // [q]
+    const int* lay_off;        // [q]
+    const int* lay_moff;       // [q]
+    const int16_t* steps;      // [n_steps][2]
+    const int* grp_off;        // [n_steps + 1]
+    const collision_group* groups;
+    const int16_t* beta;       // [n_param_iters][q] offsets, or null (plain: 1)
+    const int16_t* alpha;      // [n_param_iters][q] scales in 64ths, or null (off)
+    int n_param_iters;
+    int N, K, R, q, n_steps, lanes, n_slots, lt_alloc, max_trials, cmax;
+};
+
+int ldpc_lds_bytes(int K, int R, int n_slots);
+hipError_t launch_ldpc_decode(const ldpc_args& a, int batch, hipStream_t stream);
+
+struct demap_args {
+    const float* sym;          // [B][xfec][2]
+    int16_t* llr;              // [B][N]
+    const float* pts;          // [2 n_points] device (APSK), or null
+    int xfec, dlen, N, n_mod, n_points;
+    uint32_t col[8];
+    float qpsk_scale, psk8_scale, gain;
+};
+hipError_t launch_demap(const demap_args& a, int batch, hipStream_t stream);
+
+struct bch_args {
+    const uint8_t* hard;       // [B][n_bytes]
+    int32_t* valid;            // [B]
+    const uint64_t* lut;       // [256][W]
+    const uint64_t* pw;        // [64][P][W]
+    int n_bytes, degree, chunk;
+};
+hipError_t launch_bch_check(const bch_args& a, int batch, hipStream_t stream);
+
+}  // namespace cusat

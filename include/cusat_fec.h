@@ -1,0 +1,138 @@
+/*
+ * cusat_fec.h -- C ABI of the MI355X (gfx950) RX FEC hot path:
+ * soft demap + bit deinterleave -> int16 LLRs -> layered offset min-sum LDPC
+ * over the DVB-S2/S2X IRA codes -> BCH syndrome check.
+ *
+ * Plain C: pointers, sizes and a HIP stream handle (hipStream_t, passed as
+ * void*; NULL = the null stream).  Every call returns CUSAT_OK (0) or a
+ * negative CUSAT_E_* code; cusat_last_error() describes the last failure of
+ * the calling thread.  No call aborts the process.
+ *
+ * Each entry point names the reference (wineslab/cusat) interface it
+ * replaces.  One context owns the device tables and the per-batch message
+ * workspace of one (frame size, code rate, modulation); calls on one
+ * context must be ordered on one stream (as one reference decoder object
+ * is used from one stream, rx_fec_chain.cc:105-118).
+ */
+#ifndef CUSAT_FEC_H
+#define CUSAT_FEC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Values follow the enums of include/cusat_dsp/modcod_types.h. */
+enum cusat_framesize { CUSAT_FECFRAME_NORMAL = 0, CUSAT_FECFRAME_SHORT = 1, CUSAT_FECFRAME_MEDIUM = 2 };
+enum cusat_modulation {
+    CUSAT_MOD_QPSK = 0, CUSAT_MOD_8PSK, CUSAT_MOD_8APSK, CUSAT_MOD_16APSK,
+    CUSAT_MOD_32APSK, CUSAT_MOD_64APSK, CUSAT_MOD_128APSK, CUSAT_MOD_256APSK
+};
+enum cusat_rate {
+    CUSAT_RATE_1_4 = 0, CUSAT_RATE_1_3, CUSAT_RATE_2_5, CUSAT_RATE_1_2, CUSAT_RATE_3_5,
+    CUSAT_RATE_2_3, CUSAT_RATE_3_4, CUSAT_RATE_4_5, CUSAT_RATE_5_6, CUSAT_RATE_8_9,
+    CUSAT_RATE_9_10,
+    CUSAT_RATE_11_45, CUSAT_RATE_4_15, CUSAT_RATE_13_45, CUSAT_RATE_14_45, CUSAT_RATE_9_20,
+    CUSAT_RATE_7_15, CUSAT_RATE_90_180, CUSAT_RATE_8_15, CUSAT_RATE_96_180, CUSAT_RATE_11_20,
+    CUSAT_RATE_100_180, CUSAT_RATE_26_45, CUSAT_RATE_104_180, CUSAT_RATE_18_30,
+    CUSAT_RATE_28_45, CUSAT_RATE_23_36, CUSAT_RATE_116_180, CUSAT_RATE_20_30,
+    CUSAT_RATE_124_180, CUSAT_RATE_25_36, CUSAT_RATE_32_45, CUSAT_RATE_128_180,
+    CUSAT_RATE_13_18, CUSAT_RATE_22_30, CUSAT_RATE_132_180, CUSAT_RATE_135_180,
+    CUSAT_RATE_7_9, CUSAT_RATE_140_180, CUSAT_RATE_154_180,
+    CUSAT_RATE_COUNT_S2X = 40
+};
+
+enum {
+    CUSAT_OK = 0,
+    CUSAT_E_INVALID = -1,    /* bad argument / unsupported (frame, rate, modulation) */
+    CUSAT_E_HIP = -2,        /* HIP runtime error */
+    CUSAT_E_NOMEM = -3,      /* device or host allocation failed */
+    CUSAT_E_BATCH = -4,      /* batch size outside 1..max_batch */
+    CUSAT_E_NODEVICE = -5    /* no usable gfx950 device */
+};
+
+/* Context flags */
+#define CUSAT_CTX_SEQUENTIAL_SCHEDULE 0x1u  /* one layer per step, as CUSAT_LDPC_SEQ=1
+                                               (ldpc_decode_kernel.cu:1121-1125) */
+
+typedef struct cusat_fec_ctx cusat_fec_ctx;
+
+typedef struct cusat_fec_info {
+    int n_ldpc, k_ldpc;        /* LDPC N, K (= n_bch) */
+    int k_bch, n_bch, bch_t;   /* BCH code (fec/code_params.cc:17-108) */
+    int bch_degree;            /* deg g(x) */
+    int q, n_steps, lanes;     /* layers, schedule steps, layers per step (1|2) */
+    int cnl;                   /* max info edges per check (links_max_cn - 2) */
+    int lt_alloc;              /* message entries per frame = sum_i 360 (cnt_i + 2) */
+    int n_mod;                 /* bits per symbol */
+    int xfec_len;              /* symbols per XFECFRAME */
+    int demap_len;             /* symbols demapped (128APSK: 9258 of 9270) */
+    int max_batch, max_trials;
+    int n_collision_slots;     /* LDS delta slots the schedule needs */
+    int lds_bytes;             /* dynamic LDS of the decode kernel */
+} cusat_fec_info;
+
+const char* cusat_last_error(void);
+
+/* Replaces the constructors of ldpc_decode_cuda (ldpc_decode_kernel.cu:990),
+ * demapper_cuda (demapper_kernel.cu:188) and bch_check_cuda
+ * (bch_check_kernel.cu:13) for one MODCOD; allocates all device state for
+ * max_batch frames. */
+int cusat_fec_ctx_create(cusat_fec_ctx** out, int framesize, int rate, int modulation,
+                         int max_trials, int max_batch, unsigned flags);
+int cusat_fec_ctx_destroy(cusat_fec_ctx* ctx);
+int cusat_fec_ctx_info(const cusat_fec_ctx* ctx, cusat_fec_info* info);
+/* The same dimensions without a device or any allocation (host tables only);
+ * max_batch / max_trials are reported as 0. */
+int cusat_fec_query(int framesize, int rate, int modulation, unsigned flags, cusat_fec_info* info);
+
+/* Demapper constants the kernel uses for (noise_var, llr_gain):
+ * QPSK scale 2 sqrt(2)/noise_var, 8PSK scale 2 sin(pi/8) g, and the capped
+ * gain g = min(llr_gain, 2*320/dmax) (demapper_kernel.cu:53-75,283-291). */
+int cusat_fec_demap_scales(const cusat_fec_ctx* ctx, float noise_var, float llr_gain,
+                           float* qpsk_scale, float* psk8_scale, float* gain);
+/* Constellation points (re, im) by label [2 << n_mod floats] and the
+ * deinterleaver column offsets [8] the demapper uses. */
+int cusat_fec_constellation(const cusat_fec_ctx* ctx, float* points, uint32_t* col_offsets);
+
+/* Replaces demapper_cuda::demap_batch_async (demapper_kernel.cu:335-362):
+ * sym_dev = B x xfec_len complex float (device), llr_dev = B x n_ldpc int16
+ * (device).  The caller owns both buffers; no copy is made. */
+int cusat_fec_demap(cusat_fec_ctx* ctx, const float* sym_dev, int16_t* llr_dev, int batch,
+                    float noise_var, float llr_gain, void* stream);
+
+/* Replaces ldpc_decode_cuda::decode_launch_async (ldpc_decode_kernel.cu:1324-1336):
+ * llr_dev = B x n_ldpc int16 (device), hard_out = B x k_ldpc/8 bytes (MSB first),
+ * trials_out = B ints (remaining trials at convergence, -1 = failed).
+ * Output pointers may be device or pinned host memory. */
+int cusat_ldpc_decode(cusat_fec_ctx* ctx, const int16_t* llr_dev, uint8_t* hard_out,
+                      int32_t* trials_out, int batch, void* stream);
+
+/* Replaces bch_check_cuda::check_async (bch_check_kernel.cu:26-36):
+ * hard = B x n_bch/8 bytes, valid_out = B ints (1 = c(x) mod g(x) == 0). */
+int cusat_bch_check(cusat_fec_ctx* ctx, const uint8_t* hard, int32_t* valid_out, int batch,
+                    void* stream);
+
+/* The per-micro-batch device sequence of rx_fec_chain::launch_slot_gpu
+ * (rx_fec_chain.cc:400-481): demap -> LDPC -> BCH on one stream.  llr_dev is a
+ * caller buffer of B x n_ldpc int16 (device). */
+int cusat_rx_fec_batch(cusat_fec_ctx* ctx, const float* sym_dev, int16_t* llr_dev,
+                       int batch, float noise_var, float llr_gain, uint8_t* hard_out,
+                       int32_t* trials_out, int32_t* valid_out, void* stream);
+
+/* ---- test-traffic generator (host CPU) ----
+ * Seeded DVB-S2 transmit chain: uniform info bits (std::mt19937) -> BCH
+ * encode (fec/bch_code.cc:124-143) -> IRA LDPC encode
+ * (ldpc_encode_kernel.cu:1-17) -> bit interleave + map (mapper_kernel.cu:22-51)
+ * -> AWGN at es_n0_db with sigma = sqrt(N0/2), N0 = 10^(-Es/N0/10).
+ * sym_out = B x xfec_len complex float; info_out = B x k_bch/8 bytes (or NULL);
+ * cw_out = B x n_ldpc/8 bytes codeword (or NULL).  Noise is skipped when
+ * es_n0_db >= 1000. */
+int cusat_gen_frames(int framesize, int rate, int modulation, uint64_t seed, int batch,
+                     float es_n0_db, float* sym_out, uint8_t* info_out, uint8_t* cw_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
