@@ -76,6 +76,9 @@ def lib():
         L.cusat_bch_check.argtypes = [vp, vp, vp, i, vp]
         L.cusat_rx_fec_batch.argtypes = [vp, vp, vp, i, f, f, vp, vp, vp, vp]
         L.cusat_gen_frames.argtypes = [i, i, i, ctypes.c_uint64, i, f, vp, vp, vp]
+        L.cusat_modcod_constellation.argtypes = [i, i, i, vp, vp, ctypes.POINTER(i),
+                                                 ctypes.POINTER(i)]
+        L.cusat_bch_encode.argtypes = [i, i, vp, vp, i]
         _lib = L
     return _lib
 
@@ -174,3 +177,25 @@ def geometry(framesize, rate, modulation, flags=0):
     _check(lib().cusat_fec_query(framesize, rate_id(rate), mod_id(modulation), flags,
                                  ctypes.byref(inf)), "cusat_fec_query")
     return inf.as_dict()
+
+
+def modcod_constellation(framesize, rate, modulation):
+    """Host tables of a MODCOD: (points float32 [2 << n_mod], column offsets
+    uint32 [8], bits per symbol, interleaver rows)."""
+    pts = np.zeros(512, np.float32)
+    col = np.zeros(8, np.uint32)
+    nm, rows = ctypes.c_int(), ctypes.c_int()
+    _check(lib().cusat_modcod_constellation(framesize, rate_id(rate), mod_id(modulation), _ptr(pts),
+                                            _ptr(col), ctypes.byref(nm), ctypes.byref(rows)),
+           "cusat_modcod_constellation")
+    return pts[: 2 << nm.value].copy(), col, nm.value, rows.value
+
+
+def bch_encode(framesize, rate, msg):
+    """Systematic BCH encode of uint8 [B, k_bch/8] messages (host)."""
+    g = geometry(framesize, rate, 0)
+    msg = np.ascontiguousarray(msg, np.uint8).reshape(-1, g["k_bch"] // 8)
+    cw = np.zeros((msg.shape[0], g["n_bch"] // 8), np.uint8)
+    _check(lib().cusat_bch_encode(framesize, rate_id(rate), _ptr(msg), _ptr(cw), msg.shape[0]),
+           "cusat_bch_encode")
+    return cw

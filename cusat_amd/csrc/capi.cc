@@ -36,7 +36,8 @@ struct cusat_fec_ctx {
     int framesize, rate, modulation, max_trials, max_batch;
     fec_params fp{};
     decoder_geometry geo;
-    int cmax = 0;
+    int cmax = 0;      // kernel variant: info circulants per layer
+    bool exact = false;
     constellation con;
     interleaver il;
     int xfec = 0, dlen = 0;
@@ -52,7 +53,7 @@ struct cusat_fec_ctx {
     collision_group* d_groups = nullptr;
     float* d_pts = nullptr;
     uint64_t *d_bch_lut = nullptr, *d_bch_pw = nullptr;
-    int8_t* d_msg = nullptr;  // max_batch x lt_alloc
+    int8_t* d_msg = nullptr;  // max_batch x ldpc_msg_bytes(q, cmax)
 };
 
 namespace {
@@ -144,6 +145,7 @@ ldpc_args make_ldpc_args(const cusat_fec_ctx* c, const int16_t* llr, uint8_t* ha
     a.lt_alloc = c->geo.lt_alloc;
     a.max_trials = c->max_trials;
     a.cmax = c->cmax;
+    a.exact = c->exact;
     return a;
 }
 
@@ -200,9 +202,12 @@ int prepare(cusat_fec_ctx* c, int framesize, int rate, int modulation, unsigned 
         set_last_error("geometry: %s", e.what());
         return CUSAT_E_INVALID;
     }
-    c->cmax = 0;
-    for (int v : c->geo.lay_cnt) c->cmax = std::max(c->cmax, v);
-    if (c->cmax > 28 || c->geo.n_slots > 63) {
+    int cmin = 1 << 30, cmx = 0;
+    for (int v : c->geo.lay_cnt) {
+        cmin = std::min(cmin, v);
+        cmx = std::max(cmx, v);
+    }
+    if (!ldpc_select_variant(cmin, cmx, c->geo.lanes, &c->cmax, &c->exact) || c->geo.n_slots > 63) {
         set_last_error("code exceeds kernel limits");
         return CUSAT_E_INVALID;
     }
@@ -272,7 +277,7 @@ extern "C" int cusat_fec_ctx_create(cusat_fec_ctx** out, int framesize, int rate
     hipGetDevice(&c->device);
     rc = upload_tables(c);
     if (rc == CUSAT_OK) {
-        hipError_t e = hipMalloc(&c->d_msg, (size_t)max_batch * c->geo.lt_alloc);
+        hipError_t e = hipMalloc(&c->d_msg, (size_t)max_batch * ldpc_msg_bytes(c->geo.q, c->cmax));
         if (e != hipSuccess) {
             set_last_error("message workspace: %s", hipGetErrorString(e));
             rc = CUSAT_E_NOMEM;
@@ -375,4 +380,35 @@ extern "C" int cusat_rx_fec_batch(cusat_fec_ctx* c, const float* sym, int16_t* l
     if (rc == CUSAT_OK) rc = cusat_ldpc_decode(c, llr, hard, trials, batch, stream);
     if (rc == CUSAT_OK) rc = cusat_bch_check(c, hard, valid, batch, stream);
     return rc;
+}
+
+extern "C" int cusat_modcod_constellation(int framesize, int rate, int modulation, float* points,
+                                          uint32_t* col, int* n_mod, int* rows)
+{
+    constellation con;
+    interleaver il;
+    if (!make_constellation(modulation, rate, framesize, &con) ||
+        !make_interleaver(modulation, rate, framesize, &il)) {
+        set_last_error("unsupported (frame %d, rate %d, modulation %d)", framesize, rate, modulation);
+        return CUSAT_E_INVALID;
+    }
+    if (points) memcpy(points, con.xy.data(), sizeof(float) * 2 * con.n_points);
+    if (col) for (int b = 0; b < 8; b++) col[b] = il.col[b];
+    if (n_mod) *n_mod = con.bits_per_sym;
+    if (rows) *rows = il.rows;
+    return CUSAT_OK;
+}
+
+extern "C" int cusat_bch_encode(int framesize, int rate, const uint8_t* msg, uint8_t* cw, int batch)
+{
+    fec_params fp;
+    bch_code bch;
+    if (!msg || !cw || batch < 0 || !get_fec_params(framesize, rate, &fp) ||
+        !make_bch(framesize, fp.n_bch, fp.t, &bch) || bch.k_bits != fp.k_bch) {
+        set_last_error("cusat_bch_encode: bad arguments");
+        return CUSAT_E_INVALID;
+    }
+    for (int b = 0; b < batch; b++)
+        bch_encode(bch, msg + (size_t)b * (fp.k_bch / 8), cw + (size_t)b * (fp.n_bch / 8));
+    return CUSAT_OK;
 }

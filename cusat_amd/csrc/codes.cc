@@ -274,7 +274,8 @@ decoder_geometry build_geometry(const ldpc_table& t, bool use_schedule)
                 int id = g.lay_off[i] + k;
                 auto it = slot_of.find(id);
                 uint32_t sl = it == slot_of.end() ? 0u : (uint32_t)(it->second + 1);
-                g.circ[id] = (uint32_t)lay[i][k].second | ((uint32_t)lay[i][k].first << 9) | (sl << 18);
+                const uint32_t sh = (uint32_t)lay[i][k].second, r = (uint32_t)lay[i][k].first;
+                g.circ[id] = (sh % 180u) | ((sh >= 180u ? 1u : 0u) << 8) | ((180u * r) << 9) | (sl << 24);
             }
         }
         g.grp_off.push_back((int)g.groups.size());

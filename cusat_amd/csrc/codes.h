@@ -52,7 +52,8 @@ struct decoder_geometry {
     std::vector<int> lay_cnt;    // [q] info circulants per layer
     std::vector<int> lay_off;    // [q] first circulant of the layer
     std::vector<int> lay_moff;   // [q] message offset of the layer in a frame
-    // packed circulant: bits 0-8 shift s, 9-17 row r, 18-23 slot+1 (0 = direct)
+    // packed circulant: bits 0-7 s mod 180, bit 8 s >= 180, bits 9-23 the
+    // LDS dword base 180 r of VN group r, bits 24-29 slot+1 (0 = direct)
     std::vector<uint32_t> circ;
     std::vector<int16_t> steps;  // [n_steps][2], -1 = idle lane
     std::vector<int> grp_off;    // [n_steps + 1]

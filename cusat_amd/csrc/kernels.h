@@ -11,7 +11,7 @@ namespace cusat {
 
 struct ldpc_args {
     const int16_t* llr;        // [B][N]
-    int8_t* msg;               // [B][lt_alloc] check-to-variable messages
+    int8_t* msg;               // [B][ldpc_msg_bytes] check-to-variable message records
     uint8_t* hard;             // [B][K/8]
     int32_t* result;           // [B]
     const uint32_t* circ;      // packed circulants, per layer
@@ -25,10 +25,19 @@ struct ldpc_args {
     const int16_t* alpha;      // [n_param_iters][q] scales in 64ths, or null (off)
     int n_param_iters;
     int N, K, R, q, n_steps, lanes, n_slots, lt_alloc, max_trials, cmax;
+    bool exact;
 };
 
 int ldpc_lds_bytes(int K, int R, int n_slots);
+// Kernel variant for a code: CMAX info circulants per layer (exact when every
+// layer has cnt == CMAX, else guarded), lanes 1|2.  Returns false when no
+// variant fits.
+bool ldpc_select_variant(int cnt_min, int cnt_max, int lanes, int* cmax, bool* exact);
+// message record bytes per frame for a variant
+size_t ldpc_msg_bytes(int q, int cmax);
 hipError_t launch_ldpc_decode(const ldpc_args& a, int batch, hipStream_t stream);
+template <int CMAX, bool EXACT, int LANES>
+hipError_t launch_ldpc_variant(const ldpc_args& a, int batch, hipStream_t stream);
 
 struct demap_args {
     const float* sym;          // [B][xfec][2]

@@ -1,0 +1,403 @@
+// Layered offset/normalised min-sum LDPC decode kernel for the DVB-S2/S2X
+// IRA codes on gfx950 (device code; instantiated in ldpc_inst_*.hip).
+//
+// Semantics follow lib/cusat_dsp/cuda/ldpc_decode_kernel.cu (wineslab/cusat):
+// int16 APP with saturation, int8 check-to-variable messages clamped to
+// +-127, offset beta (plain = 1) inside the check node (:242-275), optional
+// scale n/64 (:75-90), data writes that re-apply the delta to the APP
+// (:793-801), parity writes from the read-phase value (:803-833), syndrome
+// early exit before every iteration (:689-749), result = T - t or -1.
+//
+// Work split (one workgroup = one frame):
+//  * A lane (one of the two layers of a schedule step) is 3 wave64s; thread
+//    t < 180 of a lane owns checks j = t and j = t + 180 of the lane's layer
+//    and evaluates both in the two halves of packed int16 registers
+//    (v_pk_sub_i16 clamp = qsat16, v_pk_max_i16(x, sat(0 - x)) = qabs,
+//    v_pk_sub_u16 clamp = the offset qsubu, v_pk_min/max for min1/min2).
+//    Lanes are whole waves, so branches on the layer id are wave-uniform and
+//    the schedule / circulant tables are read with scalar loads.
+//
+// LDS (int16 APP, ~130 KB for a normal frame):
+//  * VN group r (VNs 360 r .. 360 r + 359) is 180 dwords; dword x holds VN
+//    360 r + x (low half) and 360 r + x + 180 (high half).  Check j of layer
+//    i reads VN 360 r + (j - s) mod 360 for each circulant (r, s) of the
+//    layer, so the pair (t, t + 180) reads ONE dword, (t - s) mod 180, with
+//    its halves swapped when (t - s) mod 360 >= 180.  A wave reads 64
+//    consecutive dwords (one wrap at most): no bank conflicts.
+//  * Parity VNs of layer i are 180 dwords: dword x = (P(i, x), P(i, x+180)),
+//    P(i, j) = channel bit K + q j + i.  No swap is ever needed.
+//  * Delta slots [n_slots][180] dwords in the same interleaved layout.
+//
+// Messages (HBM): one record per (frame, layer, thread): both checks' int8
+// messages, edge-interleaved (A0 B0 A1 B1 ...), data edges first, the two
+// parity edges at slots CMAX and CMAX + 1, padded to 16-byte multiples and
+// moved with dwordx4; the record of the next step is prefetched while the
+// current step computes.
+//
+// Collisions: a step touches each VN once except on colliding circulants (a
+// weight-2/3 circulant inside a layer, or a VN group shared by the two
+// layers of a step).  Those edges park their delta (y - old) in an LDS slot;
+// after the step's barrier one thread per dword adds the deltas in
+// (lane, circulant) order with int16 saturation per add.  All reads of a
+// step therefore see the step-start APP, and a VN touched once gets exactly
+// the reference's update.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace cusat {
+namespace ldpc_dev {
+
+constexpr int LANE_THREADS = 192;  // 3 x wave64 per lane
+constexpr int HALF = 180;
+constexpr uint32_t SEL_ID = 0x03020100u;   // v_perm: keep halves
+constexpr uint32_t SEL_ROT = 0x01000302u;  // v_perm: swap halves
+
+typedef short s2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ s2 as_s2(uint32_t w) { return __builtin_bit_cast(s2, w); }
+__device__ __forceinline__ uint32_t bits(s2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ u2 as_u2(s2 v) { return __builtin_bit_cast(u2, v); }
+__device__ __forceinline__ s2 as_s2u(u2 v) { return __builtin_bit_cast(s2, v); }
+__device__ __forceinline__ s2 splat(int a)
+{
+    s2 v;
+    v.x = (short)a;
+    v.y = (short)a;
+    return v;
+}
+__device__ __forceinline__ uint32_t perm(uint32_t w, uint32_t sel)
+{
+    return __builtin_amdgcn_perm(w, w, sel);
+}
+
+// int8 message pair of edge e: bytes (A_e, B_e) at byte 2 (e & 1) of record
+// word e / 2 -> packed sign-extended int16.  e folds to a constant after
+// unrolling.
+__device__ __forceinline__ s2 unpack_msg(const uint32_t* w, int e)
+{
+    const uint32_t sel = (e & 1) ? 0x030c020cu : 0x010c000cu;  // [0, A, 0, B]
+    return as_s2(__builtin_amdgcn_perm(w[e >> 1], w[e >> 1], sel)) >> (short)8;
+}
+
+// record word of edges (2w, 2w + 1)
+__device__ __forceinline__ uint32_t pack_msg(s2 y_even, s2 y_odd)
+{
+    return __builtin_amdgcn_perm(bits(y_odd), bits(y_even), 0x06040200u);
+}
+
+__device__ __forceinline__ int nms_mag(int m, int n) { return (m * n + 32) >> 6; }
+
+// message record geometry for a kernel variant
+template <int CMAX>
+struct rec_t {
+    static constexpr int NE = CMAX + 2;
+    static constexpr int WORDS = (((NE + 1) / 2) + 3) / 4 * 4;
+    static constexpr int V4 = WORDS / 4;
+};
+
+// LDS int16 index of VN 360 r + jj
+__device__ __forceinline__ int vn_index(int v)
+{
+    const int r = v / CIRC, jj = v - r * CIRC;
+    return r * CIRC + (jj < HALF ? 2 * jj : 2 * (jj - HALF) + 1);
+}
+
+// Circulant word (host packs it, codes.cc): bits 0-7 s mod 180, bit 8
+// s >= 180, bits 9-23 dword base 180 r of the VN group, bits 24-29 slot + 1.
+struct circ_view {
+    int sp, h, base, slot;
+    __device__ __forceinline__ explicit circ_view(uint32_t c)
+        : sp((int)(c & 255u)), h((int)((c >> 8) & 1u)), base((int)((c >> 9) & 0x7fffu)),
+          slot((int)(c >> 24))
+    {
+    }
+};
+
+// dword index and half-swap selector of thread tt's VN pair on circulant c
+__device__ __forceinline__ void pair_addr(const circ_view& c, int tt, int& x, uint32_t& sel)
+{
+    const bool wrap = tt < c.sp;
+    x = tt - c.sp + (wrap ? HALF : 0);
+    sel = (wrap != (c.h != 0)) ? SEL_ROT : SEL_ID;
+}
+
+struct kparams {
+    int N, K, R, q, n_steps, max_trials, n_param_iters;
+};
+
+template <int CMAX, bool EXACT, int LANES>
+__global__ void __launch_bounds__(LANES* LANE_THREADS)
+    ldpc_kernel(const int16_t* __restrict__ llr_all, uint4* __restrict__ msg_all,
+                uint8_t* __restrict__ hard_all, int32_t* __restrict__ result,
+                const uint32_t* __restrict__ circ, const int* __restrict__ lay_cnt,
+                const int* __restrict__ lay_off, const int16_t* __restrict__ steps,
+                const int* __restrict__ grp_off, const collision_group* __restrict__ groups,
+                const int16_t* __restrict__ beta, const int16_t* __restrict__ alpha, kparams p)
+{
+    using R = rec_t<CMAX>;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    uint32_t* app32 = reinterpret_cast<uint32_t*>(lds_raw);  // K/2 dwords
+    uint32_t* par32 = app32 + p.K / 2;                        // R/2 dwords
+    uint32_t* dsl32 = par32 + p.R / 2;                        // n_slots * 180 dwords
+    int16_t* app16 = reinterpret_cast<int16_t*>(lds_raw);
+    int16_t* par16 = reinterpret_cast<int16_t*>(par32);
+
+    constexpr int NTHR = LANES * LANE_THREADS;
+    const int cw = blockIdx.x;
+    const int t = threadIdx.x;
+    const int lane = LANES == 2 ? __builtin_amdgcn_readfirstlane(t >= LANE_THREADS ? 1 : 0) : 0;
+    const int tt = t - lane * LANE_THREADS;
+    const bool act = tt < HALF;
+    const int q = p.q;
+    const int16_t* llr = llr_all + (size_t)cw * p.N;
+    uint4* rec_base = msg_all + (size_t)cw * q * HALF * R::V4;
+
+    // Load with the int16-APP input clamp (ldpc_decode_kernel.cu:655-670):
+    // -32768 -> -32767.
+    for (int v = t; v < p.K; v += NTHR) app16[vn_index(v)] = (int16_t)max((int)llr[v], -32767);
+    for (int c = t; c < p.R; c += NTHR) {
+        const int i = c % q, j = c / q;
+        par16[i * CIRC + (j < HALF ? 2 * j : 2 * (j - HALF) + 1)] = (int16_t)max((int)llr[p.K + c], -32767);
+    }
+    __syncthreads();
+
+    uint4 rn[R::V4];  // record prefetched for this thread's next step
+#pragma unroll
+    for (int k = 0; k < R::V4; k++) rn[k] = make_uint4(0, 0, 0, 0);
+
+    int res = -1;
+    for (int it = 0;; it++) {
+        // ---- syndrome over all checks, step by step, early exit ----
+        int ok = 1;
+        for (int s = 0; s < p.n_steps; s++) {
+            const int i = steps[2 * s + lane];
+            int bad = 0;
+            if (act && i >= 0) {
+                uint32_t sg = par32[i * HALF + tt];
+                if (i > 0) {
+                    sg ^= par32[(i - 1) * HALF + tt];
+                } else {
+                    const uint32_t w = par32[(q - 1) * HALF + (tt > 0 ? tt - 1 : HALF - 1)];
+                    sg ^= tt > 0 ? w : (w << 16);
+                }
+                const int cnt = EXACT ? CMAX : lay_cnt[i];
+                const uint32_t* cc = circ + lay_off[i];
+                for (int e = 0; e < cnt; e++) {
+                    const circ_view c(cc[e]);
+                    int x;
+                    uint32_t sel;
+                    pair_addr(c, tt, x, sel);
+                    sg ^= perm(app32[c.base + x], sel);
+                }
+                bad = (sg & 0x80008000u) != 0;
+            }
+            if (__syncthreads_or(bad)) { ok = 0; break; }
+        }
+        if (ok) { res = p.max_trials - it; break; }
+        if (it == p.max_trials) break;
+
+        const bool first = it == 0;
+        // ---- one layered iteration ----
+        for (int s = 0; s < p.n_steps; s++) {
+            const int i = steps[2 * s + lane];
+            uint32_t rw[R::WORDS];
+#pragma unroll
+            for (int k = 0; k < R::V4; k++) {
+                rw[4 * k] = rn[k].x;
+                rw[4 * k + 1] = rn[k].y;
+                rw[4 * k + 2] = rn[k].z;
+                rw[4 * k + 3] = rn[k].w;
+            }
+            // prefetch the record of this thread's next layer (next step, or
+            // step 0 of the next iteration)
+            {
+                const int sn = s + 1 < p.n_steps ? s + 1 : 0;
+                const int inx = steps[2 * sn + lane];
+                if (act && inx >= 0 && !(first && s + 1 < p.n_steps)) {
+                    const uint4* src = rec_base + ((size_t)inx * HALF + tt) * R::V4;
+#pragma unroll
+                    for (int k = 0; k < R::V4; k++) rn[k] = src[k];
+                }
+            }
+            if (act && i >= 0) {
+                int bt = 1, al = 0;
+                if (beta && it < p.n_param_iters) bt = beta[it * q + i];
+                if (alpha && it < p.n_param_iters) al = alpha[it * q + i];
+                const u2 beta2 = as_u2(splat(bt));
+                const int cnt = EXACT ? CMAX : lay_cnt[i];
+                const uint32_t* cc = circ + lay_off[i];
+                const s2 zero = splat(0);
+
+                s2 xv[CMAX], mv[CMAX], av[CMAX], ov[CMAX];
+                int xs[CMAX];
+                uint32_t sels[CMAX];
+                s2 mn1 = splat(32767), mn2 = splat(32767);
+                uint32_t sg = 0;
+#pragma unroll
+                for (int e = 0; e < CMAX; e++) {
+                    if (EXACT || e < cnt) {
+                        const circ_view c(cc[e]);
+                        pair_addr(c, tt, xs[e], sels[e]);
+                        const s2 ap = as_s2(perm(app32[c.base + xs[e]], sels[e]));
+                        const s2 o = first ? zero : unpack_msg(rw, e);
+                        const s2 x = __builtin_elementwise_sub_sat(ap, o);
+                        const s2 ab = __builtin_elementwise_max(x, __builtin_elementwise_sub_sat(zero, x));
+                        const s2 m = as_s2u(__builtin_elementwise_sub_sat(as_u2(ab), beta2));
+                        mn2 = __builtin_elementwise_min(mn2, __builtin_elementwise_max(mn1, m));
+                        mn1 = __builtin_elementwise_min(mn1, m);
+                        sg ^= bits(x);
+                        av[e] = ap;
+                        ov[e] = o;
+                        xv[e] = x;
+                        mv[e] = m;
+                    }
+                }
+                // parity edges: own P(i, j) and previous P(i-1, j) / P(q-1, j-1).
+                // Check (0, 0) has no second parity edge: its A half reads a
+                // +32767 placeholder that cannot change min1/min2 or the sign.
+                const int p0 = i * HALF + tt;
+                const bool dummy = i == 0 && tt == 0;
+                const int p1 = i > 0 ? (i - 1) * HALF + tt : (q - 1) * HALF + (tt > 0 ? tt - 1 : HALF - 1);
+                s2 x0, m0, x1, m1;
+                {
+                    const s2 o = first ? zero : unpack_msg(rw, CMAX);
+                    x0 = __builtin_elementwise_sub_sat(as_s2(par32[p0]), o);
+                    const s2 ab = __builtin_elementwise_max(x0, __builtin_elementwise_sub_sat(zero, x0));
+                    m0 = as_s2u(__builtin_elementwise_sub_sat(as_u2(ab), beta2));
+                    mn2 = __builtin_elementwise_min(mn2, __builtin_elementwise_max(mn1, m0));
+                    mn1 = __builtin_elementwise_min(mn1, m0);
+                    sg ^= bits(x0);
+                }
+                {
+                    const s2 o = first ? zero : unpack_msg(rw, CMAX + 1);
+                    uint32_t w = par32[p1];
+                    if (dummy) w = (w << 16) | 0x7fffu;
+                    x1 = __builtin_elementwise_sub_sat(as_s2(w), o);
+                    const s2 ab = __builtin_elementwise_max(x1, __builtin_elementwise_sub_sat(zero, x1));
+                    m1 = as_s2u(__builtin_elementwise_sub_sat(as_u2(ab), beta2));
+                    mn2 = __builtin_elementwise_min(mn2, __builtin_elementwise_max(mn1, m1));
+                    mn1 = __builtin_elementwise_min(mn1, m1);
+                    sg ^= bits(x1);
+                }
+                // outgoing magnitudes: min(scale(min), 127) per half
+                u2 c1, c2;
+                if (al == 0) {
+                    c1 = __builtin_elementwise_min(as_u2(mn1), (u2)127);
+                    c2 = __builtin_elementwise_min(as_u2(mn2), (u2)127);
+                } else {
+                    s2 a1, a2;
+                    a1.x = (short)min(nms_mag(mn1.x, al), 127);
+                    a1.y = (short)min(nms_mag(mn1.y, al), 127);
+                    a2.x = (short)min(nms_mag(mn2.x, al), 127);
+                    a2.y = (short)min(nms_mag(mn2.y, al), 127);
+                    c1 = as_u2(a1);
+                    c2 = as_u2(a2);
+                }
+                const u2 dlt = c1 - c2;
+                // y = sign(extrinsic) * (m == min1 ? c2 : c1)
+                auto out = [&](s2 m, s2 x) {
+                    const u2 ne = __builtin_elementwise_min(
+                        __builtin_elementwise_sub_sat(as_u2(m), as_u2(mn1)), (u2)1);
+                    const s2 mag = as_s2u(ne * dlt + c2);
+                    const s2 sgn = as_s2(sg ^ bits(x)) >> (short)15;
+                    return (mag ^ sgn) - sgn;
+                };
+                s2 ys[CMAX + 2];
+#pragma unroll
+                for (int e = 0; e < CMAX; e++) {
+                    ys[e] = zero;
+                    if (EXACT || e < cnt) {
+                        const s2 y = out(mv[e], xv[e]);
+                        ys[e] = y;
+                        const s2 d = y - ov[e];
+                        const circ_view c(cc[e]);
+                        if (c.slot == 0)
+                            app32[c.base + xs[e]] = perm(bits(__builtin_elementwise_add_sat(av[e], d)), sels[e]);
+                        else
+                            dsl32[(c.slot - 1) * HALF + xs[e]] = perm(bits(d), sels[e]);
+                    }
+                }
+                {
+                    const s2 y0 = out(m0, x0);
+                    par32[p0] = bits(__builtin_elementwise_add_sat(x0, y0));
+                    s2 y1 = out(m1, x1);
+                    const s2 n1 = __builtin_elementwise_add_sat(x1, y1);
+                    if (!dummy) {
+                        par32[p1] = bits(n1);
+                    } else {
+                        par16[2 * p1] = n1.y;  // low half of dword p1 = P(q-1, 179)
+                        y1.x = 0;
+                    }
+                    ys[CMAX] = y0;
+                    ys[CMAX + 1] = y1;
+                }
+                uint32_t ow[R::WORDS];
+#pragma unroll
+                for (int k = 0; k < R::WORDS; k++)
+                    ow[k] = 2 * k + 1 < CMAX + 2 ? pack_msg(ys[2 * k], ys[2 * k + 1])
+                          : 2 * k < CMAX + 2     ? pack_msg(ys[2 * k], zero)
+                                                 : 0u;
+                uint4* dst = rec_base + ((size_t)i * HALF + tt) * R::V4;
+#pragma unroll
+                for (int k = 0; k < R::V4; k++)
+                    dst[k] = make_uint4(ow[4 * k], ow[4 * k + 1], ow[4 * k + 2], ow[4 * k + 3]);
+            }
+            __syncthreads();
+            const int g0 = grp_off[s], g1 = grp_off[s + 1];
+            if (g1 > g0) {
+                const int total = (g1 - g0) * HALF;
+                for (int idx = t; idx < total; idx += NTHR) {
+                    const int gi = idx / HALF, x = idx - gi * HALF;
+                    const collision_group* cg = groups + g0 + gi;
+                    const int v = cg->row * HALF + x;
+                    const int n = cg->n;
+                    s2 val = as_s2(app32[v]);
+                    for (int e = 0; e < n; e++)
+                        val = __builtin_elementwise_add_sat(val, as_s2(dsl32[cg->slot[e] * HALF + x]));
+                    app32[v] = bits(val);
+                }
+                __syncthreads();
+            }
+        }
+    }
+
+    // hard decisions, K bits MSB first (ldpc_decode_kernel.cu:468-493)
+    uint8_t* h = hard_all + (size_t)cw * (p.K >> 3);
+    for (int b = t; b < (p.K >> 3); b += NTHR) {
+        int byte = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) byte |= (app16[vn_index(8 * b + k)] < 0 ? 1 : 0) << (7 - k);
+        h[b] = (uint8_t)byte;
+    }
+    if (t == 0) result[cw] = res;
+}
+
+}  // namespace ldpc_dev
+
+template <int CMAX, bool EXACT, int LANES>
+hipError_t launch_ldpc_variant(const ldpc_args& a, int batch, hipStream_t stream)
+{
+    auto kern = &ldpc_dev::ldpc_kernel<CMAX, EXACT, LANES>;
+    const int lds = ldpc_lds_bytes(a.K, a.R, a.n_slots);
+    static int configured = 0;
+    if (lds > __atomic_load_n(&configured, __ATOMIC_ACQUIRE)) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        if (e != hipSuccess) return e;
+        __atomic_store_n(&configured, lds, __ATOMIC_RELEASE);
+    }
+    const ldpc_dev::kparams p{a.N, a.K, a.R, a.q, a.n_steps, a.max_trials, a.n_param_iters};
+    hipLaunchKernelGGL(kern, dim3(batch), dim3(LANES * ldpc_dev::LANE_THREADS), lds, stream, a.llr,
+                       reinterpret_cast<uint4*>(a.msg), a.hard, a.result, a.circ, a.lay_cnt, a.lay_off,
+                       a.steps, a.grp_off, a.groups, a.beta, a.alpha, p);
+    return hipGetLastError();
+}
+
+#define CUSAT_LDPC_INSTANTIATE(C, E, L) \
+    template hipError_t launch_ldpc_variant<C, E, L>(const ldpc_args&, int, hipStream_t);
+
+}  // namespace cusat

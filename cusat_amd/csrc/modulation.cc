@@ -115,9 +115,20 @@ struct ring_row {
 
 bool build_64apsk(constellation* c, int rate)
 {
-    // 8+16+20+20APSK (EN 302 307-2 Table 13c/13d), rates 7/9, 4/5, 5/6;
-    // label = p X q bbb with row = (X, bbb)
-    static const ring_row ROWS[16] = {
+    // EN 302 307-2 Tables 13a-13f: three ring structures.
+    // 4+12+20+28 (label = row p q), 8+16+20+20 (label = p X q bbb),
+    // 16+16+16+16 (label = RR q p bb)
+    static const ring_row R_4_12_20_28[16] = {
+        {3, {1.0 / 4, 7.0 / 4, 3.0 / 4, 5.0 / 4}},       {3, {13.0 / 28, 43.0 / 28, 15.0 / 28, 41.0 / 28}},
+        {3, {1.0 / 28, 55.0 / 28, 27.0 / 28, 29.0 / 28}}, {0, {1.0 / 4, 7.0 / 4, 3.0 / 4, 5.0 / 4}},
+        {3, {9.0 / 28, 47.0 / 28, 19.0 / 28, 37.0 / 28}}, {3, {11.0 / 28, 45.0 / 28, 17.0 / 28, 39.0 / 28}},
+        {2, {1.0 / 20, 39.0 / 20, 19.0 / 20, 21.0 / 20}}, {1, {1.0 / 12, 23.0 / 12, 11.0 / 12, 13.0 / 12}},
+        {3, {5.0 / 28, 51.0 / 28, 23.0 / 28, 33.0 / 28}}, {2, {9.0 / 20, 31.0 / 20, 11.0 / 20, 29.0 / 20}},
+        {3, {3.0 / 28, 53.0 / 28, 25.0 / 28, 31.0 / 28}}, {1, {5.0 / 12, 19.0 / 12, 7.0 / 12, 17.0 / 12}},
+        {2, {1.0 / 4, 7.0 / 4, 3.0 / 4, 5.0 / 4}},       {2, {7.0 / 20, 33.0 / 20, 13.0 / 20, 27.0 / 20}},
+        {2, {3.0 / 20, 37.0 / 20, 17.0 / 20, 23.0 / 20}}, {1, {1.0 / 4, 7.0 / 4, 3.0 / 4, 5.0 / 4}},
+    };
+    static const ring_row R_8_16_20_20[16] = {
         {1, {25.0 / 16, 23.0 / 16, 7.0 / 16, 9.0 / 16}},  {3, {7.0 / 4, 5.0 / 4, 1.0 / 4, 3.0 / 4}},
         {1, {27.0 / 16, 21.0 / 16, 5.0 / 16, 11.0 / 16}}, {2, {7.0 / 4, 5.0 / 4, 1.0 / 4, 3.0 / 4}},
         {3, {31.0 / 20, 29.0 / 20, 9.0 / 20, 11.0 / 20}}, {3, {33.0 / 20, 27.0 / 20, 7.0 / 20, 13.0 / 20}},
@@ -127,24 +138,128 @@ bool build_64apsk(constellation* c, int rate)
         {0, {15.0 / 8, 9.0 / 8, 1.0 / 8, 7.0 / 8}},       {3, {39.0 / 20, 21.0 / 20, 1.0 / 20, 19.0 / 20}},
         {1, {31.0 / 16, 17.0 / 16, 1.0 / 16, 15.0 / 16}}, {2, {39.0 / 20, 21.0 / 20, 1.0 / 20, 19.0 / 20}},
     };
-    double g1, g2, g3;
-    if (rate == R4_5 || rate == R7_9 || rate == R140_180) { g1 = 2.2; g2 = 3.6; g3 = 5.2; }
-    else if (rate == R5_6) { g1 = 2.2; g2 = 3.5; g3 = 5.0; }
-    else return false;
-    static const int COUNT[4] = {8, 16, 20, 20};
+    static const ring_row R_16_16_16_16[16] = {
+        {0, {1.0 / 16, 31.0 / 16, 15.0 / 16, 17.0 / 16}}, {0, {3.0 / 16, 29.0 / 16, 13.0 / 16, 19.0 / 16}},
+        {0, {7.0 / 16, 25.0 / 16, 9.0 / 16, 23.0 / 16}},  {0, {5.0 / 16, 27.0 / 16, 11.0 / 16, 21.0 / 16}},
+        {1, {1.0 / 16, 31.0 / 16, 15.0 / 16, 17.0 / 16}}, {1, {3.0 / 16, 29.0 / 16, 13.0 / 16, 19.0 / 16}},
+        {1, {7.0 / 16, 25.0 / 16, 9.0 / 16, 23.0 / 16}},  {1, {5.0 / 16, 27.0 / 16, 11.0 / 16, 21.0 / 16}},
+        {3, {1.0 / 16, 31.0 / 16, 15.0 / 16, 17.0 / 16}}, {3, {3.0 / 16, 29.0 / 16, 13.0 / 16, 19.0 / 16}},
+        {3, {7.0 / 16, 25.0 / 16, 9.0 / 16, 23.0 / 16}},  {3, {5.0 / 16, 27.0 / 16, 11.0 / 16, 21.0 / 16}},
+        {2, {1.0 / 16, 31.0 / 16, 15.0 / 16, 17.0 / 16}}, {2, {3.0 / 16, 29.0 / 16, 13.0 / 16, 19.0 / 16}},
+        {2, {7.0 / 16, 25.0 / 16, 9.0 / 16, 23.0 / 16}},  {2, {5.0 / 16, 27.0 / 16, 11.0 / 16, 21.0 / 16}},
+    };
+    int layout = 0;  // 0: 4+12+20+28, 1: 8+16+20+20, 2: 16+16+16+16
+    double g1 = 2.4, g2 = 4.3, g3 = 7.0;
+    if (rate == R4_5 || rate == R7_9 || rate == R140_180) { layout = 1; g1 = 2.2; g2 = 3.6; g3 = 5.2; }
+    else if (rate == R5_6) { layout = 1; g1 = 2.2; g2 = 3.5; g3 = 5.0; }
+    else if (rate == R128_180) { layout = 2; g1 = 1.88; g2 = 2.72; g3 = 3.95; }
+    static const int COUNT[3][4] = {{4, 12, 20, 28}, {8, 16, 20, 20}, {16, 16, 16, 16}};
+    const ring_row* rows = layout == 0 ? R_4_12_20_28 : layout == 1 ? R_8_16_20_20 : R_16_16_16_16;
     const double ratio[4] = {1.0, g1, g2, g3};
     double energy = 0.0;
-    for (int i = 0; i < 4; i++) energy += COUNT[i] * ratio[i] * ratio[i];
+    for (int i = 0; i < 4; i++) energy += COUNT[layout][i] * ratio[i] * ratio[i];
     const double scale = std::sqrt(64.0 / energy);
     float radius[4];
     for (int i = 0; i < 4; i++) radius[i] = (float)(ratio[i] * scale);
     for (int row = 0; row < 16; row++)
         for (int col = 0; col < 4; col++) {
             const int p = col >> 1, q = col & 1;
-            const int label = (p << 5) | ((row >> 3) << 4) | (q << 3) | (row & 7);
-            polar(c, label, radius[ROWS[row].ring], ROWS[row].a[col] * M_PI);
+            int label;
+            if (layout == 0) label = (row << 2) | (p << 1) | q;
+            else if (layout == 1) label = (p << 5) | ((row >> 3) << 4) | (q << 3) | (row & 7);
+            else label = ((row >> 2) << 4) | (q << 3) | (p << 2) | (row & 3);
+            polar(c, label, radius[rows[row].ring], rows[row].a[col] * M_PI);
         }
     return true;
+}
+
+void build_8apsk(constellation* c, int rate)
+{
+    // 2+4+2APSK, EN 302 307-2 Tables 10a/10b
+    const double g1 = (rate == R100_180) ? 5.32 : 6.39;
+    const double g2 = (rate == R100_180) ? 6.8 : 8.0;
+    const double r1 = std::sqrt(8.0 / (2.0 + 4.0 * g1 * g1 + 2.0 * g2 * g2));
+    const double radius[8] = {r1, g1 * r1, g1 * r1, g2 * r1, r1, g1 * r1, g1 * r1, g2 * r1};
+    const double angle[8] = {1.0, 1.0 + 0.352, 1.0 - 0.352, 1.0, 0.0, -0.352, 0.352, 0.0};
+    for (int l = 0; l < 8; l++) polar(c, l, radius[l], angle[l] * M_PI);
+}
+
+void build_16apsk_8_8(constellation* c, int rate)
+{
+    // 8+8APSK: explicit coordinates (Table 11e) for 18/30 and 20/30, else the
+    // ring form of Tables 11c/11d
+    static const double C18[16][2] = {
+        {0.4718, 0.2606},  {0.2606, 0.4718},  {-0.4718, 0.2606},  {-0.2606, 0.4718},
+        {0.4718, -0.2606}, {0.2606, -0.4718}, {-0.4718, -0.2606}, {-0.2606, -0.4718},
+        {1.2088, 0.4984},  {0.4984, 1.2088},  {-1.2088, 0.4984},  {-0.4984, 1.2088},
+        {1.2088, -0.4984}, {0.4984, -1.2088}, {-1.2088, -0.4984}, {-0.4984, -1.2088},
+    };
+    static const double C20[16][2] = {
+        {0.5061, 0.2474},  {0.2474, 0.5061},  {-0.5061, 0.2474},  {-0.2474, 0.5061},
+        {0.5061, -0.2474}, {0.2474, -0.5061}, {-0.5061, -0.2474}, {-0.2474, -0.5061},
+        {1.2007, 0.4909},  {0.4909, 1.2007},  {-1.2007, 0.4909},  {-0.4909, 1.2007},
+        {1.2007, -0.4909}, {0.4909, -1.2007}, {-1.2007, -0.4909}, {-0.4909, -1.2007},
+    };
+    const double(*xy)[2] = rate == R18_30 ? C18 : rate == R20_30 ? C20 : nullptr;
+    if (xy) {
+        double energy = 0.0;
+        for (int i = 0; i < 16; i++) energy += xy[i][0] * xy[i][0] + xy[i][1] * xy[i][1];
+        const double scale = std::sqrt(16.0 / energy);
+        for (int i = 0; i < 16; i++) set(c, i, (float)(xy[i][0] * scale), (float)(xy[i][1] * scale));
+        return;
+    }
+    const double gamma = 2.19;
+    const double r1 = std::sqrt(2.0 / (1.0 + gamma * gamma));
+    static const int NN[2][2][2] = {{{1, 15}, {7, 9}}, {{3, 13}, {5, 11}}};
+    for (int l = 0; l < 16; l++) {
+        const int ring = (l >> 3) & 1, q = (l >> 2) & 1, pp = (l >> 1) & 1, b0 = l & 1;
+        polar(c, l, ring ? gamma * r1 : r1, NN[b0][pp][q] * (M_PI / 8.0));
+    }
+}
+
+void build_32apsk_rb(constellation* c, int rate, int framesize)
+{
+    // 4+12+16rbAPSK, EN 302 307-2 Tables 12a/12b; row = (b3, b2, b0), column = (b4, b1)
+    double g1 = 2.85, g2 = 5.55;
+    if (rate == R32_45) { g1 = 2.839; g2 = 5.258; }
+    else if (framesize == 1) { g1 = 2.84; g2 = 5.54; }
+    const double r1 = std::sqrt(32.0 / (4.0 + 12.0 * g1 * g1 + 16.0 * g2 * g2));
+    const double radius[4] = {0.0, r1, g1 * r1, g2 * r1};
+    static const int RING[8] = {3, 3, 2, 2, 3, 3, 2, 1};
+    static const double ANG[8][4] = {
+        {11.0 / 16, 5.0 / 16, 21.0 / 16, 27.0 / 16}, {9.0 / 16, 7.0 / 16, 23.0 / 16, 25.0 / 16},
+        {3.0 / 4, 1.0 / 4, 5.0 / 4, 7.0 / 4},         {7.0 / 12, 5.0 / 12, 17.0 / 12, 19.0 / 12},
+        {13.0 / 16, 3.0 / 16, 19.0 / 16, 29.0 / 16}, {15.0 / 16, 1.0 / 16, 17.0 / 16, 31.0 / 16},
+        {11.0 / 12, 1.0 / 12, 13.0 / 12, 23.0 / 12}, {3.0 / 4, 1.0 / 4, 5.0 / 4, 7.0 / 4},
+    };
+    for (int l = 0; l < 32; l++) {
+        const int row = (((l >> 3) & 1) << 2) | (((l >> 2) & 1) << 1) | (l & 1);
+        const int col = (((l >> 4) & 1) << 1) | ((l >> 1) & 1);
+        polar(c, l, radius[RING[row]], ANG[row][col] * M_PI);
+    }
+}
+
+void build_32apsk_4_8_4_16(constellation* c, int rate)
+{
+    // EN 302 307-2 Table 12e; row = (b4, b3, b0), column = (b2, b1)
+    double g1 = 2.6, g2 = 2.99, g3 = 5.6;
+    if (rate == R132_180) g2 = 2.86;
+    else if (rate == R7_9 || rate == R140_180) { g1 = 2.8; g2 = 3.08; }
+    else if (rate == R32_45) { g1 = 2.84; g2 = 2.84; g3 = 5.26; }
+    const double r1 = std::sqrt(32.0 / (4.0 + 8.0 * g1 * g1 + 4.0 * g2 * g2 + 16.0 * g3 * g3));
+    const double radius[5] = {0.0, r1, g1 * r1, g2 * r1, g3 * r1};
+    static const int RING[8] = {1, 4, 2, 4, 2, 4, 3, 4};
+    static const double ANG[8][4] = {
+        {1.0 / 4, 7.0 / 4, 3.0 / 4, 5.0 / 4},         {7.0 / 16, 25.0 / 16, 9.0 / 16, 23.0 / 16},
+        {1.0 / 12, 23.0 / 12, 11.0 / 12, 13.0 / 12}, {1.0 / 16, 31.0 / 16, 15.0 / 16, 17.0 / 16},
+        {5.0 / 12, 19.0 / 12, 7.0 / 12, 17.0 / 12},  {5.0 / 16, 27.0 / 16, 11.0 / 16, 21.0 / 16},
+        {1.0 / 4, 7.0 / 4, 3.0 / 4, 5.0 / 4},         {3.0 / 16, 29.0 / 16, 13.0 / 16, 19.0 / 16},
+    };
+    for (int l = 0; l < 32; l++) {
+        const int row = (((l >> 4) & 1) << 2) | (((l >> 3) & 1) << 1) | (l & 1);
+        const int col = (((l >> 2) & 1) << 1) | ((l >> 1) & 1);
+        polar(c, l, radius[RING[row]], ANG[row][col] * M_PI);
+    }
 }
 
 void build_128apsk(constellation* c, int rate)
@@ -294,7 +409,6 @@ const char* column_order(int m, int r)
 
 bool make_constellation(int modulation, int rate, int framesize, constellation* c)
 {
-    (void)framesize;
     *c = constellation();
     switch (modulation) {
     case QPSK: {
@@ -310,15 +424,20 @@ bool make_constellation(int modulation, int rate, int framesize, constellation* 
         for (int l = 0; l < 8; l++) set(c, l, P[l][0], P[l][1]);
         break;
     }
+    case APSK8:
+        c->bits_per_sym = 3;
+        build_8apsk(c, rate);
+        break;
     case APSK16:
-        if (is_16apsk_8_8(rate)) return false;
         c->bits_per_sym = 4;
-        build_16apsk(c, rate);
+        if (is_16apsk_8_8(rate)) build_16apsk_8_8(c, rate);
+        else build_16apsk(c, rate);
         break;
     case APSK32:
-        if (is_32apsk_other(rate)) return false;
         c->bits_per_sym = 5;
-        build_32apsk(c, rate);
+        if (rate == R2_3 || rate == R32_45) build_32apsk_rb(c, rate, framesize);
+        else if (is_32apsk_other(rate)) build_32apsk_4_8_4_16(c, rate);
+        else build_32apsk(c, rate);
         break;
     case APSK64:
         c->bits_per_sym = 6;

@@ -121,6 +121,18 @@ int cusat_rx_fec_batch(cusat_fec_ctx* ctx, const float* sym_dev, int16_t* llr_de
                        int batch, float noise_var, float llr_gain, uint8_t* hard_out,
                        int32_t* trials_out, int32_t* valid_out, void* stream);
 
+/* Host-only (no device): the constellation [2 << n_mod floats] and
+ * deinterleaver column offsets [8] of a MODCOD, its bits per symbol and
+ * interleaver rows (0 for QPSK).  Mirrors make_constellation /
+ * make_bit_interleaver (modulation/constellation.cc:461-493,
+ * modulation/bit_interleaver.cc:76-95). */
+int cusat_modcod_constellation(int framesize, int rate, int modulation, float* points,
+                               uint32_t* col_offsets, int* n_mod, int* rows);
+
+/* Host-only systematic BCH encode of B messages (k_bch/8 bytes each) into
+ * n_bch/8-byte codewords (fec/bch_code.cc:124-143). */
+int cusat_bch_encode(int framesize, int rate, const uint8_t* msg, uint8_t* cw, int batch);
+
 /* ---- test-traffic generator (host CPU) ----
  * Seeded DVB-S2 transmit chain: uniform info bits (std::mt19937) -> BCH
  * encode (fec/bch_code.cc:124-143) -> IRA LDPC encode

@@ -1,0 +1,206 @@
+"""Host-side tests (no GPU): the C ABI library loads and exports every
+declared symbol, the host tables agree value for value with the reference's
+own host code (oracle/_ref, when built), and the seeded generator produces
+valid, reproducible codewords."""
+import ctypes
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+import cusat_amd as ca
+from oracle import oracle_ctypes as oc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HERE = os.path.dirname(os.path.abspath(__file__))
+KAT = json.load(open(os.path.join(HERE, "golden", "kat.json")))
+REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libref_host.so")
+
+
+def declared_symbols():
+    syms = []
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", fn)).read()
+            syms += re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(cusat_\w+)\s*\(", text, re.M)
+    return sorted(set(syms))
+
+
+def test_library_exports_every_declared_symbol():
+    L = ca.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 12
+    for s in syms:
+        assert hasattr(L, s), s
+
+
+def test_errors_are_reported_not_fatal():
+    inf = ca.FecInfo()
+    rc = ca.lib().cusat_fec_query(1, ca.rate_id("9/10"), 0, 0, ctypes.byref(inf))
+    assert rc == -1
+    assert b"unsupported" in ca.lib().cusat_last_error()
+    with pytest.raises(RuntimeError):
+        ca.geometry(0, "3/4", 9)  # no such modulation
+
+
+SUPPORTED = [(0, r) for r in ["1/4", "1/3", "2/5", "1/2", "3/5", "2/3", "3/4", "4/5", "5/6", "8/9",
+                              "9/10", "13/45", "9/20", "11/20", "26/45", "28/45", "23/36", "25/36",
+                              "13/18", "7/9", "90/180", "96/180", "100/180", "104/180", "116/180",
+                              "124/180", "128/180", "132/180", "135/180", "140/180", "154/180",
+                              "18/30", "20/30", "22/30"]] + \
+            [(1, r) for r in ["1/4", "1/3", "2/5", "1/2", "3/5", "2/3", "3/4", "4/5", "5/6", "8/9",
+                              "11/45", "4/15", "14/45", "7/15", "8/15", "26/45", "32/45"]]
+
+
+@pytest.mark.parametrize("fs,rate", SUPPORTED)
+def test_query_every_code(fs, rate):
+    g = ca.geometry(fs, rate, "QPSK")
+    n = 64800 if fs == 0 else 16200
+    assert g["n_ldpc"] == n and g["n_bch"] == g["k_ldpc"]
+    assert g["k_bch"] + g["bch_degree"] == g["n_bch"]
+    assert g["lds_bytes"] <= 160 * 1024
+    assert g["q"] * 360 == n - g["k_ldpc"]
+
+
+def _ref():
+    if not os.path.exists(REF_LIB):
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    L = ctypes.CDLL(REF_LIB)
+    i, vp = ctypes.c_int, ctypes.c_void_p
+    L.ref_constellation.argtypes = [i, i, i, vp, ctypes.POINTER(i)]
+    L.ref_interleaver.argtypes = [i, i, i, vp, ctypes.POINTER(i), ctypes.POINTER(i)]
+    L.ref_fec_params.argtypes = [i, i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i)]
+    L.ref_bch.argtypes = [i, i, i, vp, vp, vp]
+    L.ref_ldpc_table.argtypes = [i, i] + [ctypes.POINTER(i)] * 5 + [vp, i]
+    return L
+
+
+@pytest.mark.parametrize("fs,rate", SUPPORTED)
+def test_fec_params_and_table_selection_vs_reference(fs, rate):
+    L = _ref()
+    k, n, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    assert L.ref_fec_params(fs, ca.rate_id(rate), ctypes.byref(k), ctypes.byref(n), ctypes.byref(t)) == 0
+    g = ca.geometry(fs, rate, "QPSK")
+    assert (g["k_bch"], g["n_bch"], g["bch_t"]) == (k.value, n.value, t.value)
+    v = [ctypes.c_int() for _ in range(5)]
+    addr = np.zeros(20000, np.uint16)
+    total = L.ref_ldpc_table(fs, ca.rate_id(rate), *[ctypes.byref(x) for x in v],
+                             addr.ctypes.data_as(ctypes.c_void_p), len(addr))
+    N, K, n_rows, lt, lmc = [x.value for x in v]
+    assert (N, K) == (g["n_ldpc"], g["k_ldpc"])
+    assert g["lt_alloc"] == lt + 1 and g["cnl"] == lmc - 2
+    # the same addresses as our embedded table text, in order
+    from tests.test_tables import TABLES
+    mine = [t for t in TABLES.values() if t["N"] == N and t["K"] == K and
+            [a for r in t["rows"] for a in r] == list(addr[:total])]
+    assert len(mine) >= 1
+
+
+MODCODS = [("QPSK", "1/2"), ("8PSK", "3/4"), ("8PSK", "3/5"), ("8PSK", "9/10"), ("16APSK", "2/3"),
+           ("16APSK", "5/6"), ("32APSK", "3/4"), ("32APSK", "9/10"), ("64APSK", "4/5"),
+           ("128APSK", "135/180"), ("256APSK", "135/180"), ("8APSK", "100/180"),
+           ("16APSK", "90/180"), ("16APSK", "18/30"), ("32APSK", "2/3"), ("32APSK", "128/180"),
+           ("64APSK", "132/180"), ("64APSK", "128/180"), ("128APSK", "140/180"),
+           ("256APSK", "116/180")]
+
+
+@pytest.mark.parametrize("mod,rate", MODCODS)
+@pytest.mark.parametrize("fs", [0, 1])
+def test_constellation_and_interleaver_vs_reference(mod, rate, fs):
+    if fs == 1 and (rate in ("9/10", "135/180") or mod in ("64APSK", "128APSK", "256APSK")):
+        pytest.skip("no short frame for this MODCOD")
+    L = _ref()
+    pts, col, nm, rows = ca.modcod_constellation(fs, rate, mod)
+    rp = np.zeros(512, np.float32)
+    bits = ctypes.c_int()
+    assert L.ref_constellation(ca.mod_id(mod), ca.rate_id(rate), fs,
+                               rp.ctypes.data_as(ctypes.c_void_p), ctypes.byref(bits)) == 0
+    assert bits.value == nm
+    np.testing.assert_array_equal(pts.view(np.uint32), rp[: len(pts)].view(np.uint32))
+    rc = np.zeros(8, np.uint32)
+    nc, rr = ctypes.c_int(), ctypes.c_int()
+    assert L.ref_interleaver(ca.mod_id(mod), ca.rate_id(rate), fs, rc.ctypes.data_as(ctypes.c_void_p),
+                             ctypes.byref(nc), ctypes.byref(rr)) == 0
+    np.testing.assert_array_equal(col, rc)
+    assert rows == rr.value
+
+
+@pytest.mark.parametrize("fs,rate", [(0, "1/2"), (0, "2/3"), (0, "3/4"), (0, "9/10"), (1, "1/2"),
+                                     (1, "8/9")])
+def test_bch_encoder_vs_reference(fs, rate):
+    L = _ref()
+    g = ca.geometry(fs, rate, "QPSK")
+    rng = np.random.default_rng(11)
+    msg = rng.integers(0, 256, (2, g["k_bch"] // 8), dtype=np.uint8)
+    mine = ca.bch_encode(fs, rate, msg)
+    gen = np.zeros(4, np.uint64)
+    for b in range(2):
+        cw = np.zeros(g["n_bch"] // 8, np.uint8)
+        p = L.ref_bch(fs, g["n_bch"], g["bch_t"], msg[b].ctypes.data_as(ctypes.c_void_p),
+                      cw.ctypes.data_as(ctypes.c_void_p), gen.ctypes.data_as(ctypes.c_void_p))
+        assert p == g["bch_degree"]
+        np.testing.assert_array_equal(mine[b], cw)
+    d, go = oc.bch_generator(fs, g["bch_t"])
+    np.testing.assert_array_equal(go, gen)
+
+
+@pytest.mark.parametrize("mod,rate,fs,order,rows", [tuple(v) for v in KAT["interleaver"]])
+def test_interleaver_kats(mod, rate, fs, order, rows):
+    _, col, nm, r = ca.modcod_constellation(fs, rate, mod)
+    if rows is not None:
+        assert r == rows
+    if order is not None:
+        assert [int(c) // r for c in col[:nm]] == order
+
+
+@pytest.mark.parametrize("mod,rate", MODCODS)
+def test_constellation_properties(mod, rate):
+    pts, _, nm, _ = ca.modcod_constellation(0, rate, mod)
+    P = pts.reshape(-1, 2).astype(np.float64)
+    assert len(P) == 1 << nm
+    assert abs((P ** 2).sum(1).mean() - 1.0) < 1e-3           # unit average energy
+    assert len({(round(x, 5), round(y, 5)) for x, y in P}) == len(P)  # distinct
+    if mod == "QPSK":
+        a = np.float32(np.sqrt(0.5))
+        np.testing.assert_array_equal(pts, np.float32([a, a, a, -a, -a, a, -a, -a]))
+
+
+@pytest.mark.parametrize("rate,g1,g2", [tuple(v) for v in KAT["apsk32_ratios"]])
+def test_32apsk_ring_ratios(rate, g1, g2):
+    """qa_constellation.cc:102-128: labels 17 / 0 / 8 sit on rings 1 / 2 / 3."""
+    pts, _, _, _ = ca.modcod_constellation(0, rate, "32APSK")
+    r = np.hypot(pts[0::2].astype(np.float64), pts[1::2].astype(np.float64))
+    assert abs(r[0] / r[17] - g1) < 1e-6 * g1 and abs(r[8] / r[17] - g2) < 1e-6 * g2
+    assert (np.abs(r - r[17]) < 1e-5).sum() == 4 and (np.abs(r - r[0]) < 1e-5).sum() == 12
+
+
+def test_generator_reproducible_and_valid():
+    s1, i1, c1 = ca.gen_frames(0, "3/4", "8PSK", 5, 6, 7.0, want_cw=True)
+    s2, i2, c2 = ca.gen_frames(0, "3/4", "8PSK", 5, 6, 7.0, want_cw=True)
+    s3, _, _ = ca.gen_frames(0, "3/4", "8PSK", 6, 6, 7.0)
+    assert np.array_equal(s1, s2) and np.array_equal(i1, i2) and not np.array_equal(s1, s3)
+    # frame b does not depend on the batch size
+    s4, i4, _ = ca.gen_frames(0, "3/4", "8PSK", 5, 2, 7.0)
+    assert np.array_equal(s4, s1[:2]) and np.array_equal(i4, i1[:2])
+    code = oc.Code("s2_b7")
+    for b in range(6):
+        bits = np.unpackbits(c1[b])
+        llr = np.where(bits == 0, 100, -100).astype(np.int16)
+        assert code.syndrome_ok(llr)
+        assert np.array_equal(c1[b, : 48408 // 8], i1[b])
+    assert list(oc.bch_check(c1[:, :6075], 48600, 0, 12)) == [1] * 6
+    # noise level: Es/N0 7 dB on unit-energy 8PSK
+    n0 = 10 ** (-0.7)
+    sn, _, _ = ca.gen_frames(0, "3/4", "8PSK", 5, 6, 1000.0)
+    var = ((s1 - sn) ** 2).sum(2).mean()
+    assert abs(var / n0 - 1) < 0.02
+
+
+def test_generator_128apsk_padding_and_stuffing():
+    sym, _, cw = ca.gen_frames(0, "135/180", "128APSK", 1, 1, 1000.0, want_cw=True)
+    g = ca.geometry(0, "135/180", "128APSK")
+    assert sym.shape[1] == 9270 and g["demap_len"] == 9258
+    pts, _, _, _ = ca.modcod_constellation(0, "135/180", "128APSK")
+    assert np.all(sym[0, 9258:] == pts.reshape(-1, 2)[127])
