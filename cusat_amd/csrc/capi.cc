@@ -49,8 +49,9 @@ struct cusat_fec_ctx {
     void* d_tables = nullptr;  // one allocation holding every table below
     uint32_t* d_circ = nullptr;
     int *d_lay_cnt = nullptr, *d_lay_off = nullptr, *d_lay_moff = nullptr, *d_grp_off = nullptr;
-    int16_t* d_steps = nullptr;
-    collision_group* d_groups = nullptr;
+    int* d_steps = nullptr;
+    int kern_steps = 0;  // schedule steps as launched (padded to even)
+    int* d_groups = nullptr;
     float* d_pts = nullptr;
     uint64_t *d_bch_lut = nullptr, *d_bch_pw = nullptr;
     int8_t* d_msg = nullptr;  // max_batch x ldpc_msg_bytes(q, cmax)
@@ -88,6 +89,26 @@ int upload_tables(cusat_fec_ctx* c)
             for (int w = 0; w < W; w++) pw[((size_t)l * c->bch.degree + b) * W + w] = powers[shift + b][w];
     }
 
+    // schedule as int32 (scalar loads), padded to an even step count with an
+    // idle step (the kernel double-buffers message records over step pairs)
+    std::vector<int> steps32(g.steps.begin(), g.steps.end());
+    std::vector<int> grp_off = g.grp_off;
+    if (g.n_steps & 1) {
+        steps32.push_back(-1);
+        steps32.push_back(-1);
+        grp_off.push_back(grp_off.back());
+    }
+    c->kern_steps = (int)steps32.size() / 2;
+    // collision groups as int32 records: LDS dword base of the VN group,
+    // slot count, slot dword bases (slots follow app + parity in LDS)
+    std::vector<int> grp32;
+    const int slot_base = (g.K + g.R) / 2;
+    for (const collision_group& cg : g.groups) {
+        int rec[8] = {180 * cg.row, cg.n, 0, 0, 0, 0, 0, 0};
+        for (int e = 0; e < cg.n; e++) rec[2 + e] = slot_base + 180 * cg.slot[e];
+        grp32.insert(grp32.end(), rec, rec + 8);
+    }
+
     struct piece {
         const void* src;
         size_t bytes;
@@ -98,9 +119,9 @@ int upload_tables(cusat_fec_ctx* c)
         {g.lay_cnt.data(), g.lay_cnt.size() * 4, (void**)&c->d_lay_cnt},
         {g.lay_off.data(), g.lay_off.size() * 4, (void**)&c->d_lay_off},
         {g.lay_moff.data(), g.lay_moff.size() * 4, (void**)&c->d_lay_moff},
-        {g.steps.data(), g.steps.size() * 2, (void**)&c->d_steps},
-        {g.grp_off.data(), g.grp_off.size() * 4, (void**)&c->d_grp_off},
-        {g.groups.data(), g.groups.size() * sizeof(collision_group), (void**)&c->d_groups},
+        {steps32.data(), steps32.size() * 4, (void**)&c->d_steps},
+        {grp_off.data(), grp_off.size() * 4, (void**)&c->d_grp_off},
+        {grp32.data(), grp32.size() * 4, (void**)&c->d_groups},
         {c->con.xy.data(), (size_t)c->con.n_points * 2 * 4, (void**)&c->d_pts},
         {lut.data(), lut.size() * 8, (void**)&c->d_bch_lut},
         {pw.data(), pw.size() * 8, (void**)&c->d_bch_pw},
@@ -139,9 +160,10 @@ ldpc_args make_ldpc_args(const cusat_fec_ctx* c, const int16_t* llr, uint8_t* ha
     a.K = c->geo.K;
     a.R = c->geo.R;
     a.q = c->geo.q;
-    a.n_steps = c->geo.n_steps;
+    a.n_steps = c->kern_steps;
     a.lanes = c->geo.lanes;
     a.n_slots = c->geo.n_slots;
+    a.n_groups = (int)c->geo.groups.size();
     a.lt_alloc = c->geo.lt_alloc;
     a.max_trials = c->max_trials;
     a.cmax = c->cmax;
@@ -237,7 +259,8 @@ void fill_info(const cusat_fec_ctx* c, cusat_fec_info* o)
     o->max_batch = c->max_batch;
     o->max_trials = c->max_trials;
     o->n_collision_slots = c->geo.n_slots;
-    o->lds_bytes = ldpc_lds_bytes(c->geo.K, c->geo.R, c->geo.n_slots);
+    o->lds_bytes = ldpc_lds_bytes(c->geo.K, c->geo.R, c->geo.n_slots, c->geo.n_steps + (c->geo.n_steps & 1),
+                                  (int)c->geo.groups.size());
 }
 
 }  // namespace

@@ -241,7 +241,7 @@ decoder_geometry build_geometry(const ldpc_table& t, bool use_schedule)
     for (int i = 0; i < q; i++)
         if (seen[i] != 1) throw std::runtime_error("schedule does not cover every layer once");
 
-    g.circ.resize(off);
+    g.circ.resize(4 * (size_t)off);
     g.grp_off.push_back(0);
     for (int s = 0; s < g.n_steps; s++) {
         // all (lane, k) circulants of the step in application order
@@ -274,8 +274,17 @@ decoder_geometry build_geometry(const ldpc_table& t, bool use_schedule)
                 int id = g.lay_off[i] + k;
                 auto it = slot_of.find(id);
                 uint32_t sl = it == slot_of.end() ? 0u : (uint32_t)(it->second + 1);
-                const uint32_t sh = (uint32_t)lay[i][k].second, r = (uint32_t)lay[i][k].first;
-                g.circ[id] = (sh % 180u) | ((sh >= 180u ? 1u : 0u) << 8) | ((180u * r) << 9) | (sl << 24);
+                // LDS: VN group r is 180 dwords at dword 180 r; dword x holds
+                // VNs (x, x + 180).  Thread t's checks (t, t + 180) read dword
+                // (t - s) mod 180, halves swapped iff (t - s) mod 360 >= 180.
+                const int sh = lay[i][k].second, r = lay[i][k].first;
+                const int sp = sh % 180;
+                const int slot_base = (g.K + g.R) / 2;  // dwords before the delta slots
+                uint32_t* cr = &g.circ[4 * (size_t)id];
+                cr[0] = (uint32_t)sp;
+                cr[1] = (uint32_t)(4 * (180 * r - sp));
+                cr[2] = sl ? (uint32_t)(4 * (slot_base + 180 * (int)(sl - 1) - sp)) : 0xffffffffu;
+                cr[3] = sh >= 180 ? 0x01000302u : 0x03020100u;  // v_perm selector without wrap
             }
         }
         g.grp_off.push_back((int)g.groups.size());

@@ -52,8 +52,9 @@ struct decoder_geometry {
     std::vector<int> lay_cnt;    // [q] info circulants per layer
     std::vector<int> lay_off;    // [q] first circulant of the layer
     std::vector<int> lay_moff;   // [q] message offset of the layer in a frame
-    // packed circulant: bits 0-7 s mod 180, bit 8 s >= 180, bits 9-23 the
-    // LDS dword base 180 r of VN group r, bits 24-29 slot+1 (0 = direct)
+    // circulant records, 4 words each: s mod 180; LDS byte offset of the VN
+    // pair dword of thread 0 (no wrap); same for its delta slot or ~0 when
+    // the circulant writes the APP directly; v_perm half-swap selector
     std::vector<uint32_t> circ;
     std::vector<int16_t> steps;  // [n_steps][2], -1 = idle lane
     std::vector<int> grp_off;    // [n_steps + 1]

@@ -14,21 +14,23 @@ struct ldpc_args {
     int8_t* msg;               // [B][ldpc_msg_bytes] check-to-variable message records
     uint8_t* hard;             // [B][K/8]
     int32_t* result;           // [B]
-    const uint32_t* circ;      // packed circulants, per layer
+    const uint32_t* circ;      // circulant records (4 dwords each), per layer
     const int* lay_cnt;        // [q]
     const int* lay_off;        // [q]
     const int* lay_moff;       // [q]
-    const int16_t* steps;      // [n_steps][2]
+    const int* steps;          // [n_steps][2], n_steps even (padded with idle steps)
     const int* grp_off;        // [n_steps + 1]
-    const collision_group* groups;
-    const int16_t* beta;       // [n_param_iters][q] offsets, or null (plain: 1)
-    const int16_t* alpha;      // [n_param_iters][q] scales in 64ths, or null (off)
+    const int* groups;         // [n_groups][8] int32 collision group records
+    const int* beta;           // [n_param_iters][q] offsets, or null (plain: 1)
+    const int* alpha;          // [n_param_iters][q] scales in 64ths, or null (off)
     int n_param_iters;
-    int N, K, R, q, n_steps, lanes, n_slots, lt_alloc, max_trials, cmax;
+    int N, K, R, q, n_steps, lanes, n_slots, n_groups, lt_alloc, max_trials, cmax;
     bool exact;
 };
 
-int ldpc_lds_bytes(int K, int R, int n_slots);
+// dynamic LDS of the decode kernel: APP + parity + delta slots + schedule,
+// group offsets and group records (n_steps as launched)
+int ldpc_lds_bytes(int K, int R, int n_slots, int n_steps, int n_groups);
 // Kernel variant for a code: CMAX info circulants per layer (exact when every
 // layer has cnt == CMAX, else guarded), lanes 1|2.  Returns false when no
 // variant fits.

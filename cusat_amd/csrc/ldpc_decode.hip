@@ -39,7 +39,10 @@ bool ldpc_select_variant(int cnt_min, int cnt_max, int lanes, int* cmax, bool* e
 
 size_t ldpc_msg_bytes(int q, int cmax) { return (size_t)q * 180 * rec_words(cmax) * 4; }
 
-int ldpc_lds_bytes(int K, int R, int n_slots) { return ((K + R) * 2 + n_slots * CIRC * 2 + 15) & ~15; }
+int ldpc_lds_bytes(int K, int R, int n_slots, int n_steps, int n_groups)
+{
+    return ((K + R) * 2 + n_slots * CIRC * 2 + 4 * (3 * n_steps + 1 + 8 * n_groups) + 15) & ~15;
+}
 
 hipError_t launch_ldpc_decode(const ldpc_args& a, int batch, hipStream_t stream)
 {

@@ -106,43 +106,62 @@ __device__ __forceinline__ int vn_index(int v)
     return r * CIRC + (jj < HALF ? 2 * jj : 2 * (jj - HALF) + 1);
 }
 
-// Circulant word (host packs it, codes.cc): bits 0-7 s mod 180, bit 8
-// s >= 180, bits 9-23 dword base 180 r of the VN group, bits 24-29 slot + 1.
-struct circ_view {
-    int sp, h, base, slot;
-    __device__ __forceinline__ explicit circ_view(uint32_t c)
-        : sp((int)(c & 255u)), h((int)((c >> 8) & 1u)), base((int)((c >> 9) & 0x7fffu)),
-          slot((int)(c >> 24))
-    {
-    }
+// Circulant record (host packs it, codes.cc), 4 dwords read with one scalar
+// load: sp = s mod 180; app_off / slot_off = LDS byte offset of the pair
+// dword for thread 0 without wrap (4 (base - sp)), slot_off < 0 when the
+// circulant writes the APP directly; sel = v_perm selector of the pair when
+// thread tt does not wrap (tt >= sp); a wrap flips it (SEL_ID <-> SEL_ROT).
+struct circ_rec {
+    int sp, app_off, slot_off;
+    uint32_t sel;
 };
+constexpr uint32_t SEL_FLIP = SEL_ID ^ SEL_ROT;
 
-// dword index and half-swap selector of thread tt's VN pair on circulant c
-__device__ __forceinline__ void pair_addr(const circ_view& c, int tt, int& x, uint32_t& sel)
+// byte offset (to add to the circulant's offsets) and half-swap selector of
+// thread tt's VN pair
+__device__ __forceinline__ void pair_addr(const circ_rec& c, int tt4, int tt, int& off, uint32_t& sel)
 {
     const bool wrap = tt < c.sp;
-    x = tt - c.sp + (wrap ? HALF : 0);
-    sel = (wrap != (c.h != 0)) ? SEL_ROT : SEL_ID;
+    off = tt4 + (wrap ? 4 * HALF : 0);
+    sel = c.sel ^ (wrap ? SEL_FLIP : 0u);
+}
+
+__device__ __forceinline__ uint32_t lds_ld(const unsigned char* base, int byte_off)
+{
+    return *reinterpret_cast<const uint32_t*>(base + byte_off);
+}
+__device__ __forceinline__ void lds_st(unsigned char* base, int byte_off, uint32_t v)
+{
+    *reinterpret_cast<uint32_t*>(base + byte_off) = v;
 }
 
 struct kparams {
-    int N, K, R, q, n_steps, max_trials, n_param_iters;
+    int N, K, R, q, n_steps, max_trials, n_param_iters, n_slots, n_groups;
 };
+
+// Collision group record (int32 x 8, host-packed): LDS dword index of the
+// VN group's first pair dword, number of slots, the slots' dword indices.
+constexpr int GROUP_WORDS = 8;
 
 template <int CMAX, bool EXACT, int LANES>
 __global__ void __launch_bounds__(LANES* LANE_THREADS)
     ldpc_kernel(const int16_t* __restrict__ llr_all, uint4* __restrict__ msg_all,
                 uint8_t* __restrict__ hard_all, int32_t* __restrict__ result,
-                const uint32_t* __restrict__ circ, const int* __restrict__ lay_cnt,
-                const int* __restrict__ lay_off, const int16_t* __restrict__ steps,
-                const int* __restrict__ grp_off, const collision_group* __restrict__ groups,
-                const int16_t* __restrict__ beta, const int16_t* __restrict__ alpha, kparams p)
+                const circ_rec* __restrict__ circ, const int* __restrict__ lay_cnt,
+                const int* __restrict__ lay_off, const int* __restrict__ steps,
+                const int* __restrict__ grp_off, const int* __restrict__ groups,
+                const int* __restrict__ beta, const int* __restrict__ alpha, kparams p)
 {
     using R = rec_t<CMAX>;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     uint32_t* app32 = reinterpret_cast<uint32_t*>(lds_raw);  // K/2 dwords
     uint32_t* par32 = app32 + p.K / 2;                        // R/2 dwords
     uint32_t* dsl32 = par32 + p.R / 2;                        // n_slots * 180 dwords
+    // schedule, group offsets and group records, copied to LDS once so that
+    // every later read is an LDS read (no vector-memory wait in the loops)
+    int* sched_l = reinterpret_cast<int*>(dsl32 + p.n_slots * HALF);  // [2 n_steps]
+    int* goff_l = sched_l + 2 * p.n_steps;                            // [n_steps + 1]
+    int* grp_l = goff_l + p.n_steps + 1;                              // [n_groups][8]
     int16_t* app16 = reinterpret_cast<int16_t*>(lds_raw);
     int16_t* par16 = reinterpret_cast<int16_t*>(par32);
 
@@ -151,10 +170,15 @@ __global__ void __launch_bounds__(LANES* LANE_THREADS)
     const int t = threadIdx.x;
     const int lane = LANES == 2 ? __builtin_amdgcn_readfirstlane(t >= LANE_THREADS ? 1 : 0) : 0;
     const int tt = t - lane * LANE_THREADS;
+    const int tt4 = 4 * tt;
     const bool act = tt < HALF;
     const int q = p.q;
     const int16_t* llr = llr_all + (size_t)cw * p.N;
     uint4* rec_base = msg_all + (size_t)cw * q * HALF * R::V4;
+
+    for (int k = t; k < 2 * p.n_steps; k += NTHR) sched_l[k] = steps[k];
+    for (int k = t; k <= p.n_steps; k += NTHR) goff_l[k] = grp_off[k];
+    for (int k = t; k < GROUP_WORDS * p.n_groups; k += NTHR) grp_l[k] = groups[k];
 
     // Load with the int16-APP input clamp (ldpc_decode_kernel.cu:655-670):
     // -32768 -> -32767.
@@ -165,16 +189,40 @@ __global__ void __launch_bounds__(LANES* LANE_THREADS)
     }
     __syncthreads();
 
-    uint4 rn[R::V4];  // record prefetched for this thread's next step
+    // Every thread zeroes its own message records (iteration 0 then reads
+    // old messages = 0 like any other iteration; each record is only ever
+    // touched by its owner thread, so program order orders these stores
+    // before the loads).
+    const int tl = act ? tt : HALF - 1;  // idle lanes load a valid record they ignore
+    for (int s = 0; s < p.n_steps; s++) {
+        const int i = __builtin_amdgcn_readfirstlane(sched_l[2 * s + lane]);
+        if (act && i >= 0) {
+            uint4* dst = rec_base + ((size_t)i * HALF + tt) * R::V4;
 #pragma unroll
-    for (int k = 0; k < R::V4; k++) rn[k] = make_uint4(0, 0, 0, 0);
+            for (int k = 0; k < R::V4; k++) dst[k] = make_uint4(0, 0, 0, 0);
+        }
+    }
+    // Double-buffered message records: even steps read ra and prefetch the
+    // next step into rb, odd steps the reverse (the host pads the schedule
+    // to an even step count).  The prefetch is unconditional, so the only
+    // vector-memory wait in a step is for the record loaded one step earlier.
+    uint4 ra[R::V4], rb[R::V4];
+    {
+        const int i0 = max(__builtin_amdgcn_readfirstlane(sched_l[lane]), 0);
+        const uint4* src = rec_base + ((size_t)i0 * HALF + tl) * R::V4;
+#pragma unroll
+        for (int k = 0; k < R::V4; k++) {
+            ra[k] = src[k];
+            rb[k] = make_uint4(0, 0, 0, 0);
+        }
+    }
 
     int res = -1;
     for (int it = 0;; it++) {
         // ---- syndrome over all checks, step by step, early exit ----
         int ok = 1;
         for (int s = 0; s < p.n_steps; s++) {
-            const int i = steps[2 * s + lane];
+            const int i = __builtin_amdgcn_readfirstlane(sched_l[2 * s + lane]);
             int bad = 0;
             if (act && i >= 0) {
                 uint32_t sg = par32[i * HALF + tt];
@@ -185,13 +233,13 @@ __global__ void __launch_bounds__(LANES* LANE_THREADS)
                     sg ^= tt > 0 ? w : (w << 16);
                 }
                 const int cnt = EXACT ? CMAX : lay_cnt[i];
-                const uint32_t* cc = circ + lay_off[i];
+                const circ_rec* cc = circ + lay_off[i];
                 for (int e = 0; e < cnt; e++) {
-                    const circ_view c(cc[e]);
-                    int x;
+                    const circ_rec c = cc[e];
+                    int off;
                     uint32_t sel;
-                    pair_addr(c, tt, x, sel);
-                    sg ^= perm(app32[c.base + x], sel);
+                    pair_addr(c, tt4, tt, off, sel);
+                    sg ^= perm(lds_ld(lds_raw, c.app_off + off), sel);
                 }
                 bad = (sg & 0x80008000u) != 0;
             }
@@ -200,28 +248,30 @@ __global__ void __launch_bounds__(LANES* LANE_THREADS)
         if (ok) { res = p.max_trials - it; break; }
         if (it == p.max_trials) break;
 
-        const bool first = it == 0;
         // ---- one layered iteration ----
-        for (int s = 0; s < p.n_steps; s++) {
-            const int i = steps[2 * s + lane];
+        for (int s0 = 0; s0 < p.n_steps; s0 += 2) {
+#pragma unroll
+        for (int hs = 0; hs < 2; hs++) {
+            const int s = s0 + hs;
+            uint4* cur = hs == 0 ? ra : rb;
+            uint4* nxt = hs == 0 ? rb : ra;
+            const int i = __builtin_amdgcn_readfirstlane(sched_l[2 * s + lane]);
             uint32_t rw[R::WORDS];
 #pragma unroll
             for (int k = 0; k < R::V4; k++) {
-                rw[4 * k] = rn[k].x;
-                rw[4 * k + 1] = rn[k].y;
-                rw[4 * k + 2] = rn[k].z;
-                rw[4 * k + 3] = rn[k].w;
+                rw[4 * k] = cur[k].x;
+                rw[4 * k + 1] = cur[k].y;
+                rw[4 * k + 2] = cur[k].z;
+                rw[4 * k + 3] = cur[k].w;
             }
             // prefetch the record of this thread's next layer (next step, or
             // step 0 of the next iteration)
             {
                 const int sn = s + 1 < p.n_steps ? s + 1 : 0;
-                const int inx = steps[2 * sn + lane];
-                if (act && inx >= 0 && !(first && s + 1 < p.n_steps)) {
-                    const uint4* src = rec_base + ((size_t)inx * HALF + tt) * R::V4;
+                const int inx = max(__builtin_amdgcn_readfirstlane(sched_l[2 * sn + lane]), 0);
+                const uint4* src = rec_base + ((size_t)inx * HALF + tl) * R::V4;
 #pragma unroll
-                    for (int k = 0; k < R::V4; k++) rn[k] = src[k];
-                }
+                for (int k = 0; k < R::V4; k++) nxt[k] = src[k];
             }
             if (act && i >= 0) {
                 int bt = 1, al = 0;
@@ -229,21 +279,25 @@ __global__ void __launch_bounds__(LANES* LANE_THREADS)
                 if (alpha && it < p.n_param_iters) al = alpha[it * q + i];
                 const u2 beta2 = as_u2(splat(bt));
                 const int cnt = EXACT ? CMAX : lay_cnt[i];
-                const uint32_t* cc = circ + lay_off[i];
+                const circ_rec* cc = circ + lay_off[i];
                 const s2 zero = splat(0);
 
                 s2 xv[CMAX], mv[CMAX], av[CMAX], ov[CMAX];
-                int xs[CMAX];
+                int wa[CMAX];        // LDS byte address of the edge's write (APP or delta slot)
                 uint32_t sels[CMAX];
+                uint32_t slotm = 0;  // edges that write a delta slot (wave-uniform)
                 s2 mn1 = splat(32767), mn2 = splat(32767);
                 uint32_t sg = 0;
 #pragma unroll
                 for (int e = 0; e < CMAX; e++) {
                     if (EXACT || e < cnt) {
-                        const circ_view c(cc[e]);
-                        pair_addr(c, tt, xs[e], sels[e]);
-                        const s2 ap = as_s2(perm(app32[c.base + xs[e]], sels[e]));
-                        const s2 o = first ? zero : unpack_msg(rw, e);
+                        const circ_rec c = cc[e];
+                        int off;
+                        pair_addr(c, tt4, tt, off, sels[e]);
+                        const s2 ap = as_s2(perm(lds_ld(lds_raw, c.app_off + off), sels[e]));
+                        wa[e] = (c.slot_off < 0 ? c.app_off : c.slot_off) + off;
+                        slotm |= (c.slot_off < 0 ? 0u : 1u) << e;
+                        const s2 o = unpack_msg(rw, e);
                         const s2 x = __builtin_elementwise_sub_sat(ap, o);
                         const s2 ab = __builtin_elementwise_max(x, __builtin_elementwise_sub_sat(zero, x));
                         const s2 m = as_s2u(__builtin_elementwise_sub_sat(as_u2(ab), beta2));
@@ -264,7 +318,7 @@ __global__ void __launch_bounds__(LANES* LANE_THREADS)
                 const int p1 = i > 0 ? (i - 1) * HALF + tt : (q - 1) * HALF + (tt > 0 ? tt - 1 : HALF - 1);
                 s2 x0, m0, x1, m1;
                 {
-                    const s2 o = first ? zero : unpack_msg(rw, CMAX);
+                    const s2 o = unpack_msg(rw, CMAX);
                     x0 = __builtin_elementwise_sub_sat(as_s2(par32[p0]), o);
                     const s2 ab = __builtin_elementwise_max(x0, __builtin_elementwise_sub_sat(zero, x0));
                     m0 = as_s2u(__builtin_elementwise_sub_sat(as_u2(ab), beta2));
@@ -273,7 +327,7 @@ __global__ void __launch_bounds__(LANES* LANE_THREADS)
                     sg ^= bits(x0);
                 }
                 {
-                    const s2 o = first ? zero : unpack_msg(rw, CMAX + 1);
+                    const s2 o = unpack_msg(rw, CMAX + 1);
                     uint32_t w = par32[p1];
                     if (dummy) w = (w << 16) | 0x7fffu;
                     x1 = __builtin_elementwise_sub_sat(as_s2(w), o);
@@ -297,12 +351,18 @@ __global__ void __launch_bounds__(LANES* LANE_THREADS)
                     c1 = as_u2(a1);
                     c2 = as_u2(a2);
                 }
-                const u2 dlt = c1 - c2;
-                // y = sign(extrinsic) * (m == min1 ? c2 : c1)
+                // y = sign(extrinsic) * (m == min1 ? c2 : c1): m >= min1, so
+                // (min1 + 1) -| m is 1 exactly on the arg-min edge; its
+                // negation is the half-word mask that picks c2 (v_bfi)
+                // keep the sign parity in one register (stops re-association
+                // of the XOR chain into every edge)
+                __asm__("" : "+v"(sg));
+                const u2 mn1p1 = as_u2(mn1) + (u2)1;
+                const uint32_t c1b = bits(as_s2u(c1)), c2b = bits(as_s2u(c2));
                 auto out = [&](s2 m, s2 x) {
-                    const u2 ne = __builtin_elementwise_min(
-                        __builtin_elementwise_sub_sat(as_u2(m), as_u2(mn1)), (u2)1);
-                    const s2 mag = as_s2u(ne * dlt + c2);
+                    const u2 hit = __builtin_elementwise_sub_sat(mn1p1, as_u2(m));
+                    const uint32_t mask = bits(as_s2u((u2)0 - hit));
+                    const s2 mag = as_s2((mask & c2b) | (~mask & c1b));
                     const s2 sgn = as_s2(sg ^ bits(x)) >> (short)15;
                     return (mag ^ sgn) - sgn;
                 };
@@ -314,11 +374,10 @@ __global__ void __launch_bounds__(LANES* LANE_THREADS)
                         const s2 y = out(mv[e], xv[e]);
                         ys[e] = y;
                         const s2 d = y - ov[e];
-                        const circ_view c(cc[e]);
-                        if (c.slot == 0)
-                            app32[c.base + xs[e]] = perm(bits(__builtin_elementwise_add_sat(av[e], d)), sels[e]);
+                        if ((slotm >> e) & 1u)
+                            lds_st(lds_raw, wa[e], perm(bits(d), sels[e]));
                         else
-                            dsl32[(c.slot - 1) * HALF + xs[e]] = perm(bits(d), sels[e]);
+                            lds_st(lds_raw, wa[e], perm(bits(__builtin_elementwise_add_sat(av[e], d)), sels[e]));
                     }
                 }
                 {
@@ -347,21 +406,27 @@ __global__ void __launch_bounds__(LANES* LANE_THREADS)
                     dst[k] = make_uint4(ow[4 * k], ow[4 * k + 1], ow[4 * k + 2], ow[4 * k + 3]);
             }
             __syncthreads();
-            const int g0 = grp_off[s], g1 = grp_off[s + 1];
+            const int g0 = __builtin_amdgcn_readfirstlane(goff_l[s]);
+            const int g1 = __builtin_amdgcn_readfirstlane(goff_l[s + 1]);
             if (g1 > g0) {
-                const int total = (g1 - g0) * HALF;
-                for (int idx = t; idx < total; idx += NTHR) {
-                    const int gi = idx / HALF, x = idx - gi * HALF;
-                    const collision_group* cg = groups + g0 + gi;
-                    const int v = cg->row * HALF + x;
-                    const int n = cg->n;
-                    s2 val = as_s2(app32[v]);
-                    for (int e = 0; e < n; e++)
-                        val = __builtin_elementwise_add_sat(val, as_s2(dsl32[cg->slot[e] * HALF + x]));
-                    app32[v] = bits(val);
+                // one group per lane at a time (group id wave-uniform); thread
+                // tt adds the deltas of pair dword tt in slot order
+                for (int gb = g0; gb < g1; gb += LANES) {
+                    const int g = gb + lane;
+                    if (g < g1 && act) {
+                        const int* gr = grp_l + GROUP_WORDS * g;
+                        const int base = __builtin_amdgcn_readfirstlane(gr[0]);
+                        const int n = __builtin_amdgcn_readfirstlane(gr[1]);
+                        s2 val = as_s2(app32[base + tt]);
+                        for (int e = 0; e < n; e++)
+                            val = __builtin_elementwise_add_sat(
+                                val, as_s2(app32[__builtin_amdgcn_readfirstlane(gr[2 + e]) + tt]));
+                        app32[base + tt] = bits(val);
+                    }
                 }
                 __syncthreads();
             }
+        }
         }
     }
 
@@ -382,7 +447,7 @@ template <int CMAX, bool EXACT, int LANES>
 hipError_t launch_ldpc_variant(const ldpc_args& a, int batch, hipStream_t stream)
 {
     auto kern = &ldpc_dev::ldpc_kernel<CMAX, EXACT, LANES>;
-    const int lds = ldpc_lds_bytes(a.K, a.R, a.n_slots);
+    const int lds = ldpc_lds_bytes(a.K, a.R, a.n_slots, a.n_steps, a.n_groups);
     static int configured = 0;
     if (lds > __atomic_load_n(&configured, __ATOMIC_ACQUIRE)) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
@@ -390,9 +455,11 @@ hipError_t launch_ldpc_variant(const ldpc_args& a, int batch, hipStream_t stream
         if (e != hipSuccess) return e;
         __atomic_store_n(&configured, lds, __ATOMIC_RELEASE);
     }
-    const ldpc_dev::kparams p{a.N, a.K, a.R, a.q, a.n_steps, a.max_trials, a.n_param_iters};
+    const ldpc_dev::kparams p{a.N, a.K, a.R, a.q, a.n_steps, a.max_trials, a.n_param_iters,
+                              a.n_slots, a.n_groups};
     hipLaunchKernelGGL(kern, dim3(batch), dim3(LANES * ldpc_dev::LANE_THREADS), lds, stream, a.llr,
-                       reinterpret_cast<uint4*>(a.msg), a.hard, a.result, a.circ, a.lay_cnt, a.lay_off,
+                       reinterpret_cast<uint4*>(a.msg), a.hard, a.result,
+                       reinterpret_cast<const ldpc_dev::circ_rec*>(a.circ), a.lay_cnt, a.lay_off,
                        a.steps, a.grp_off, a.groups, a.beta, a.alpha, p);
     return hipGetLastError();
 }
