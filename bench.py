@@ -134,6 +134,13 @@ def main():
                    "sample": "%d of the same frames (LLRs from the GPU demapper), T=%d, "
                              "oracle C decoder on %d threads, %.1f s" % (n, T, nthr, dt)}
 
+    # measured HBM bytes per LDPC launch (rocprofv3 FETCH_SIZE / WRITE_SIZE of
+    # this workload, calibrated against a copy kernel; see profiles/)
+    traffic = None
+    tj = os.path.join(ROOT, "profiles", "ldpc_kernel_traffic.json")
+    if os.path.exists(tj) and (B, T, es) == (1024, 50, 7.0):
+        traffic = json.load(open(tj)).get("hbm_bytes_per_launch")
+
     out = {
         "metric": "decoded info Gbit/s at FER 1e-3, learned vs plain min-sum, 1 MI355X",
         "value": round(value, 4),
@@ -157,7 +164,7 @@ def main():
                    "oracle_parity_frames": args.check, "oracle_parity": parity},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None},
+                     "traffic": traffic},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
