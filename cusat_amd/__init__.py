@@ -199,3 +199,10 @@ def bch_encode(framesize, rate, msg):
     _check(lib().cusat_bch_encode(framesize, rate_id(rate), _ptr(msg), _ptr(cw), msg.shape[0]),
            "cusat_bch_encode")
     return cw
+
+
+def frame_passes_fec_checks(trials, bch_valid):
+    """Frame acceptance of rx_fec_chain::frame_passes_fec_checks
+    (lib/cusat_dsp/rx_fec_chain.h:193-196): the LDPC decoder converged
+    (trials >= 0) and the BCH syndrome is zero.  Element-wise on arrays."""
+    return (np.asarray(trials) >= 0) & (np.asarray(bch_valid) != 0)

@@ -204,3 +204,10 @@ def test_generator_128apsk_padding_and_stuffing():
     assert sym.shape[1] == 9270 and g["demap_len"] == 9258
     pts, _, _, _ = ca.modcod_constellation(0, "135/180", "128APSK")
     assert np.all(sym[0, 9258:] == pts.reshape(-1, 2)[127])
+
+
+def test_frame_acceptance_truth_table():
+    """lib/qa_rx_fec_chain.cc:9-17: accept only a converged LDPC with a valid BCH."""
+    t = np.array([25, 0, -1, 3, -1])
+    v = np.array([1, 1, 1, 0, 0])
+    assert list(ca.frame_passes_fec_checks(t, v)) == [True, True, False, False, False]
