@@ -79,6 +79,13 @@ def lib():
         L.cusat_modcod_constellation.argtypes = [i, i, i, vp, vp, ctypes.POINTER(i),
                                                  ctypes.POINTER(i)]
         L.cusat_bch_encode.argtypes = [i, i, vp, vp, i]
+        L.cusat_ldpc_set_params.argtypes = [vp, vp, vp, i]
+        L.cusat_ldpc_decode_learned.argtypes = [vp, vp, vp, vp, i, i, vp]
+        L.cusat_rx_fec_learned.argtypes = [vp, vp, vp, vp, vp, vp, i, i, vp]
+        L.cusat_learn_ctx_create.argtypes = [ctypes.POINTER(vp), vp, i, i]
+        L.cusat_learn_ctx_destroy.argtypes = [vp]
+        L.cusat_learn_fwd_bwd.argtypes = [vp, vp, vp, i, vp, vp, f, vp, vp, vp, vp, vp, vp]
+        L.cusat_learn_adam.argtypes = [vp, vp, vp, vp, i, f, f, f, f, i, f, f, vp]
         _lib = L
     return _lib
 
@@ -156,6 +163,58 @@ class FecContext:
         _check(lib().cusat_rx_fec_batch(self._h, _ptr(sym), _ptr(llr), batch, noise_var, llr_gain,
                                         _ptr(hard), _ptr(trials), _ptr(valid), _stream(stream)),
                "cusat_rx_fec_batch")
+
+    def set_params(self, beta, alpha):
+        """Learned check-node parameters: int16 [n_iters, q] offsets (LLR
+        units) and scales (n/64, 0 = none); empty arrays clear them."""
+        beta = np.ascontiguousarray(beta, np.int16)
+        alpha = np.ascontiguousarray(alpha, np.int16)
+        n = 0 if beta.size == 0 else beta.reshape(-1, self.info["q"]).shape[0]
+        assert alpha.shape == beta.shape
+        _check(lib().cusat_ldpc_set_params(self._h, _ptr(beta) if n else None,
+                                           _ptr(alpha) if n else None, n), "cusat_ldpc_set_params")
+
+    def decode_learned(self, llr, hard, trials, batch, max_trials, stream=None):
+        _check(lib().cusat_ldpc_decode_learned(self._h, _ptr(llr), _ptr(hard), _ptr(trials), batch,
+                                               max_trials, _stream(stream)),
+               "cusat_ldpc_decode_learned")
+
+    def rx_fec_learned(self, llr, hard, trials, valid, fallback, batch, learned_trials, stream=None):
+        _check(lib().cusat_rx_fec_learned(self._h, _ptr(llr), _ptr(hard), _ptr(trials), _ptr(valid),
+                                          _ptr(fallback), batch, learned_trials, _stream(stream)),
+               "cusat_rx_fec_learned")
+
+
+class LearnContext:
+    """fp32 forward/backward state for training the learned parameters of
+    one code over n_iters unrolled iterations (cusat_learn_* of the C ABI)."""
+
+    def __init__(self, fec, max_batch, n_iters):
+        self.fec, self.max_batch, self.n_iters = fec, max_batch, n_iters
+        h = ctypes.c_void_p()
+        _check(lib().cusat_learn_ctx_create(ctypes.byref(h), fec._h, max_batch, n_iters),
+               "cusat_learn_ctx_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cusat_learn_ctx_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def fwd_bwd(self, llr, cw, batch, alpha, beta, kappa, grad_alpha, grad_beta, loss,
+                app_out=None, grad_llr=None, stream=None):
+        _check(lib().cusat_learn_fwd_bwd(self._h, _ptr(llr), _ptr(cw), batch, _ptr(alpha), _ptr(beta),
+                                         kappa, _ptr(grad_alpha), _ptr(grad_beta), _ptr(loss),
+                                         _ptr(app_out), _ptr(grad_llr), _stream(stream)),
+               "cusat_learn_fwd_bwd")
+
+
+def adam_step(param, grad, m, v, lr, step, lo, hi, b1=0.9, b2=0.999, eps=1e-8, stream=None):
+    """In-place Adam update of a device float tensor (cusat_learn_adam)."""
+    _check(lib().cusat_learn_adam(_ptr(param), _ptr(grad), _ptr(m), _ptr(v), param.numel(), lr, b1, b2,
+                                  eps, step, lo, hi, _stream(stream)), "cusat_learn_adam")
 
 
 def gen_frames(framesize, rate, modulation, seed, batch, es_n0_db, want_cw=False):

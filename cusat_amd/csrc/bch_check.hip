@@ -25,7 +25,7 @@ __global__ void __launch_bounds__(256) bch_check_kernel(bch_args a, int batch)
 
     const int lane = threadIdx.x & 63;
     const int frame = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (frame >= batch) return;
+    if (frame >= batch || (a.active && a.active[frame] == 0)) return;
     const int P = a.degree;
     const int top_w = (P - 8) >> 6, top_b = (P - 8) & 63;
     const uint64_t top_mask = (P & 63) ? ((1ull << (P & 63)) - 1) : ~0ull;
@@ -70,7 +70,22 @@ __global__ void __launch_bounds__(256) bch_check_kernel(bch_args a, int batch)
     }
 }
 
+__global__ void __launch_bounds__(256) fallback_mask_kernel(const int32_t* trials, const int32_t* valid,
+                                                              int32_t* mask, int batch)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < batch) mask[b] = (trials[b] >= 0 && valid[b] == 1) ? 0 : 1;
+}
+
 }  // namespace
+
+hipError_t launch_fallback_mask(const int32_t* trials, const int32_t* valid, int32_t* mask, int batch,
+                                hipStream_t stream)
+{
+    hipLaunchKernelGGL(fallback_mask_kernel, dim3((batch + 255) / 256), dim3(256), 0, stream, trials, valid,
+                       mask, batch);
+    return hipGetLastError();
+}
 
 hipError_t launch_bch_check(const bch_args& a, int batch, hipStream_t stream)
 {

@@ -12,6 +12,7 @@
 
 #include "bch.h"
 #include "codes.h"
+#include "ctx.h"
 #include "cusat_fec.h"
 #include "kernels.h"
 #include "modulation.h"
@@ -32,30 +33,6 @@ void set_last_error(const char* fmt, ...)
 
 using namespace cusat;
 
-struct cusat_fec_ctx {
-    int framesize, rate, modulation, max_trials, max_batch;
-    fec_params fp{};
-    decoder_geometry geo;
-    int cmax = 0;      // kernel variant: info circulants per layer
-    bool exact = false;
-    constellation con;
-    interleaver il;
-    int xfec = 0, dlen = 0;
-    float dmax = 0, llr_target = 320.0f;
-    bch_code bch;
-    int bch_chunk = 0;
-    int device = 0;
-    // device state
-    void* d_tables = nullptr;  // one allocation holding every table below
-    uint32_t* d_circ = nullptr;
-    int *d_lay_cnt = nullptr, *d_lay_off = nullptr, *d_lay_moff = nullptr, *d_grp_off = nullptr;
-    int* d_steps = nullptr;
-    int kern_steps = 0;  // schedule steps as launched (padded to even)
-    int* d_groups = nullptr;
-    float* d_pts = nullptr;
-    uint64_t *d_bch_lut = nullptr, *d_bch_pw = nullptr;
-    int8_t* d_msg = nullptr;  // max_batch x ldpc_msg_bytes(q, cmax)
-};
 
 namespace {
 
@@ -168,6 +145,19 @@ ldpc_args make_ldpc_args(const cusat_fec_ctx* c, const int16_t* llr, uint8_t* ha
     a.max_trials = c->max_trials;
     a.cmax = c->cmax;
     a.exact = c->exact;
+    return a;
+}
+
+bch_args make_bch_args(const cusat_fec_ctx* c, const uint8_t* hard, int32_t* valid)
+{
+    bch_args a{};
+    a.hard = hard;
+    a.valid = valid;
+    a.lut = c->d_bch_lut;
+    a.pw = c->d_bch_pw;
+    a.n_bytes = c->fp.n_bch / 8;
+    a.degree = c->bch.degree;
+    a.chunk = c->bch_chunk;
     return a;
 }
 
@@ -301,6 +291,7 @@ extern "C" int cusat_fec_ctx_create(cusat_fec_ctx** out, int framesize, int rate
     rc = upload_tables(c);
     if (rc == CUSAT_OK) {
         hipError_t e = hipMalloc(&c->d_msg, (size_t)max_batch * ldpc_msg_bytes(c->geo.q, c->cmax));
+        if (e == hipSuccess) e = hipMalloc(&c->d_mask, (size_t)max_batch * sizeof(int32_t));
         if (e != hipSuccess) {
             set_last_error("message workspace: %s", hipGetErrorString(e));
             rc = CUSAT_E_NOMEM;
@@ -319,6 +310,8 @@ extern "C" int cusat_fec_ctx_destroy(cusat_fec_ctx* c)
     if (!c) return CUSAT_OK;
     if (c->d_tables) hipFree(c->d_tables);
     if (c->d_msg) hipFree(c->d_msg);
+    if (c->d_mask) hipFree(c->d_mask);
+    if (c->d_beta) hipFree(c->d_beta);
     delete c;
     return CUSAT_OK;
 }
@@ -384,14 +377,7 @@ extern "C" int cusat_bch_check(cusat_fec_ctx* c, const uint8_t* hard, int32_t* v
 {
     if (!c || !hard || !valid) { set_last_error("bad argument"); return CUSAT_E_INVALID; }
     if (!check_batch(c, batch)) return CUSAT_E_BATCH;
-    bch_args a{};
-    a.hard = hard;
-    a.valid = valid;
-    a.lut = c->d_bch_lut;
-    a.pw = c->d_bch_pw;
-    a.n_bytes = c->fp.n_bch / 8;
-    a.degree = c->bch.degree;
-    a.chunk = c->bch_chunk;
+    bch_args a = make_bch_args(c, hard, valid);
     HIP_TRY(launch_bch_check(a, batch, (hipStream_t)stream));
     return CUSAT_OK;
 }
@@ -403,6 +389,82 @@ extern "C" int cusat_rx_fec_batch(cusat_fec_ctx* c, const float* sym, int16_t* l
     if (rc == CUSAT_OK) rc = cusat_ldpc_decode(c, llr, hard, trials, batch, stream);
     if (rc == CUSAT_OK) rc = cusat_bch_check(c, hard, valid, batch, stream);
     return rc;
+}
+
+extern "C" int cusat_ldpc_set_params(cusat_fec_ctx* c, const int16_t* beta, const int16_t* alpha, int n_iters)
+{
+    if (!c || n_iters < 0 || n_iters > 1000 || (n_iters > 0 && (!beta || !alpha))) {
+        set_last_error("bad argument");
+        return CUSAT_E_INVALID;
+    }
+    const int q = c->geo.q;
+    const size_t n = (size_t)n_iters * q;
+    for (size_t k = 0; k < n; k++)
+        if (beta[k] < 0 || alpha[k] < 0 || alpha[k] > 1023) {
+            set_last_error("parameter %zu out of range (beta %d, alpha %d)", k, beta[k], alpha[k]);
+            return CUSAT_E_INVALID;
+        }
+    if (n_iters > c->param_cap) {
+        int* d = nullptr;
+        HIP_TRY(hipMalloc(&d, 2 * n * sizeof(int)));
+        if (c->d_beta) hipFree(c->d_beta);
+        c->d_beta = d;
+        c->d_alpha = d + n;
+        c->param_cap = n_iters;
+    }
+    if (n_iters > 0) {
+        // beta block then alpha block, at the current capacity's layout
+        c->d_alpha = c->d_beta + (size_t)c->param_cap * q;
+        std::vector<int> h((size_t)c->param_cap * q * 2, 0);
+        for (size_t k = 0; k < n; k++) {
+            h[k] = beta[k];
+            h[(size_t)c->param_cap * q + k] = alpha[k];
+        }
+        HIP_TRY(hipMemcpy(c->d_beta, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+    c->n_param_iters = n_iters;
+    return CUSAT_OK;
+}
+
+extern "C" int cusat_ldpc_decode_learned(cusat_fec_ctx* c, const int16_t* llr, uint8_t* hard, int32_t* trials,
+                                         int batch, int max_trials, void* stream)
+{
+    if (!c || !llr || !hard || !trials || max_trials < 0 || max_trials > 1000) {
+        set_last_error("bad argument");
+        return CUSAT_E_INVALID;
+    }
+    if (!check_batch(c, batch)) return CUSAT_E_BATCH;
+    ldpc_args a = make_ldpc_args(c, llr, hard, trials);
+    a.max_trials = max_trials;
+    if (c->n_param_iters > 0) {
+        a.beta = c->d_beta;
+        a.alpha = c->d_alpha;
+        a.n_param_iters = c->n_param_iters;
+    }
+    HIP_TRY(launch_ldpc_decode(a, batch, (hipStream_t)stream));
+    return CUSAT_OK;
+}
+
+extern "C" int cusat_rx_fec_learned(cusat_fec_ctx* c, const int16_t* llr, uint8_t* hard, int32_t* trials,
+                                    int32_t* valid, int32_t* fallback, int batch, int learned_trials,
+                                    void* stream)
+{
+    if (!c || !valid) { set_last_error("bad argument"); return CUSAT_E_INVALID; }
+    int rc = cusat_ldpc_decode_learned(c, llr, hard, trials, batch, learned_trials, stream);
+    if (rc == CUSAT_OK) rc = cusat_bch_check(c, hard, valid, batch, stream);
+    if (rc != CUSAT_OK) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    int32_t* mask = fallback ? fallback : c->d_mask;
+    HIP_TRY(launch_fallback_mask(trials, valid, mask, batch, st));
+    // plain decoder (beta = 1, no scale, max_trials of the context) on the
+    // frames the learned pass did not deliver, then their BCH check
+    ldpc_args a = make_ldpc_args(c, llr, hard, trials);
+    a.active = mask;
+    HIP_TRY(launch_ldpc_decode(a, batch, st));
+    bch_args b = make_bch_args(c, hard, valid);
+    b.active = mask;
+    HIP_TRY(launch_bch_check(b, batch, st));
+    return CUSAT_OK;
 }
 
 extern "C" int cusat_modcod_constellation(int framesize, int rate, int modulation, float* points,

@@ -212,6 +212,8 @@ decoder_geometry build_geometry(const ldpc_table& t, bool use_schedule)
         moff += CIRC * (g.lay_cnt[i] + 2);
     }
     g.lt_alloc = moff;
+    for (int i = 0; i < q; i++)
+        for (auto& rs : lay[i]) g.circ_rs.push_back((rs.first << 16) | rs.second);
 
     std::vector<std::array<int, 2>> sched;
     if (use_schedule) sched = find_schedule(t.name, q);

@@ -56,6 +56,7 @@ struct decoder_geometry {
     // pair dword of thread 0 (no wrap); same for its delta slot or ~0 when
     // the circulant writes the APP directly; v_perm half-swap selector
     std::vector<uint32_t> circ;
+    std::vector<int> circ_rs;    // the same circulants as (row << 16) | shift
     std::vector<int16_t> steps;  // [n_steps][2], -1 = idle lane
     std::vector<int> grp_off;    // [n_steps + 1]
     std::vector<collision_group> groups;

@@ -1,0 +1,44 @@
+// Private: the context behind the opaque cusat_fec_ctx handle of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "bch.h"
+#include "codes.h"
+#include "modulation.h"
+
+namespace cusat {
+void set_last_error(const char* fmt, ...);
+}
+
+struct cusat_fec_ctx {
+    int framesize, rate, modulation, max_trials, max_batch;
+    cusat::fec_params fp{};
+    cusat::decoder_geometry geo;
+    int cmax = 0;      // kernel variant: info circulants per layer
+    bool exact = false;
+    cusat::constellation con;
+    cusat::interleaver il;
+    int xfec = 0, dlen = 0;
+    float dmax = 0, llr_target = 320.0f;
+    cusat::bch_code bch;
+    int bch_chunk = 0;
+    int device = 0;
+    // device state
+    void* d_tables = nullptr;  // one allocation holding every table below
+    uint32_t* d_circ = nullptr;
+    int *d_lay_cnt = nullptr, *d_lay_off = nullptr, *d_lay_moff = nullptr, *d_grp_off = nullptr;
+    int* d_steps = nullptr;
+    int kern_steps = 0;  // schedule steps as launched (padded to even)
+    int* d_groups = nullptr;
+    float* d_pts = nullptr;
+    uint64_t *d_bch_lut = nullptr, *d_bch_pw = nullptr;
+    int8_t* d_msg = nullptr;  // max_batch x ldpc_msg_bytes(q, cmax)
+    int32_t* d_mask = nullptr;  // [max_batch] fallback mask of cusat_rx_fec_learned
+    // learned check-node parameters [n_param_iters][q] (int32 on the device)
+    int* d_beta = nullptr;
+    int* d_alpha = nullptr;
+    int n_param_iters = 0, param_cap = 0;
+};

@@ -24,6 +24,7 @@ struct ldpc_args {
     const int* beta;           // [n_param_iters][q] offsets, or null (plain: 1)
     const int* alpha;          // [n_param_iters][q] scales in 64ths, or null (off)
     int n_param_iters;
+    const int32_t* active;     // [B] nonzero = decode the frame, or null (all)
     int N, K, R, q, n_steps, lanes, n_slots, n_groups, lt_alloc, max_trials, cmax;
     bool exact;
 };
@@ -56,8 +57,13 @@ struct bch_args {
     int32_t* valid;            // [B]
     const uint64_t* lut;       // [256][W]
     const uint64_t* pw;        // [64][P][W]
+    const int32_t* active;     // [B] nonzero = check the frame, or null (all)
     int n_bytes, degree, chunk;
 };
 hipError_t launch_bch_check(const bch_args& a, int batch, hipStream_t stream);
+// mask[b] = 1 unless trials[b] >= 0 and valid[b] == 1 (a frame the plain
+// decoder must redo); optional per-frame flags (trials / valid may be pinned host)
+hipError_t launch_fallback_mask(const int32_t* trials, const int32_t* valid, int32_t* mask,
+                                int batch, hipStream_t stream);
 
 }  // namespace cusat

@@ -150,7 +150,8 @@ __global__ void __launch_bounds__(LANES* LANE_THREADS)
                 const circ_rec* __restrict__ circ, const int* __restrict__ lay_cnt,
                 const int* __restrict__ lay_off, const int* __restrict__ steps,
                 const int* __restrict__ grp_off, const int* __restrict__ groups,
-                const int* __restrict__ beta, const int* __restrict__ alpha, kparams p)
+                const int* __restrict__ beta, const int* __restrict__ alpha,
+                const int32_t* __restrict__ active, kparams p)
 {
     using R = rec_t<CMAX>;
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -167,6 +168,9 @@ __global__ void __launch_bounds__(LANES* LANE_THREADS)
 
     constexpr int NTHR = LANES * LANE_THREADS;
     const int cw = blockIdx.x;
+    // frames masked off (active[cw] == 0) keep their outputs; the whole
+    // workgroup leaves before its first barrier
+    if (active && active[cw] == 0) return;
     const int t = threadIdx.x;
     const int lane = LANES == 2 ? __builtin_amdgcn_readfirstlane(t >= LANE_THREADS ? 1 : 0) : 0;
     const int tt = t - lane * LANE_THREADS;
@@ -460,7 +464,7 @@ hipError_t launch_ldpc_variant(const ldpc_args& a, int batch, hipStream_t stream
     hipLaunchKernelGGL(kern, dim3(batch), dim3(LANES * ldpc_dev::LANE_THREADS), lds, stream, a.llr,
                        reinterpret_cast<uint4*>(a.msg), a.hard, a.result,
                        reinterpret_cast<const ldpc_dev::circ_rec*>(a.circ), a.lay_cnt, a.lay_off,
-                       a.steps, a.grp_off, a.groups, a.beta, a.alpha, p);
+                       a.steps, a.grp_off, a.groups, a.beta, a.alpha, a.active, p);
     return hipGetLastError();
 }
 

@@ -1,0 +1,243 @@
+"""Learned offset / normalised min-sum: training and evaluation driver.
+
+Training (fp32, HIP forward/backward kernels of cusat_learn_fwd_bwd):
+per-iteration x per-layer offsets beta[t][i] and scales alpha[t][i] of the
+check node, over n_iters unrolled layered iterations, BCE of every
+iteration's APP against the transmitted codeword.  Data: seeded random
+codewords from the host transmit chain (BCH -> IRA LDPC -> interleave ->
+map), AWGN added on the device at an Es/N0 drawn per sub-batch, and the
+int16 LLRs of the product demapper (cusat_fec_demap) -- the plain pipeline.
+
+Inference quantises the parameters to the integer decoder's hooks:
+beta -> round, clamped to 0..255 (LLR units; plain = 1); alpha -> n/64 with
+n = round(64 alpha) clamped to 1..1023 (the decoder's odd-symmetric
+(|v| n + 32) >> 6 scale, lib/cusat_dsp/cuda/ldpc_decode_kernel.cu:75-90).
+
+    python -m cusat_amd.learn train --out cusat_amd/data/learned/s2_b6_16apsk.json
+    python -m cusat_amd.learn eval  --params cusat_amd/data/learned/s2_b6_16apsk.json
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_PARAMS = os.path.join(HERE, "data", "learned", "s2_b6_16apsk.json")
+
+
+def quantize(alpha, beta):
+    """fp32 (alpha, beta) -> int16 (beta, alpha64) for cusat_ldpc_set_params."""
+    b = np.clip(np.rint(np.asarray(beta, np.float64)), 0, 255).astype(np.int16)
+    a = np.clip(np.rint(64.0 * np.asarray(alpha, np.float64)), 1, 1023).astype(np.int16)
+    return b, a
+
+
+def load_params(path=DEFAULT_PARAMS):
+    d = json.load(open(path))
+    return d, np.array(d["beta_q"], np.int16), np.array(d["alpha_q"], np.int16)
+
+
+def snr_scales(es_n0_db):
+    """noise_var, llr_gain as rx_fec_chain::update_snr (rx_fec_chain.h:148-156)."""
+    snr_lin = np.float32(10.0) ** (np.float32(es_n0_db) / np.float32(10.0))
+    nv = np.float32(1.0) / snr_lin
+    return float(nv), float(min(np.float32(4.0) / nv, np.float32(1000.0)))
+
+
+class FramePool:
+    """Noiseless XFECFRAMEs of seeded random codewords, resident on the
+    device, plus their codeword bits; noisy LLR batches are drawn from it."""
+
+    def __init__(self, ctx, framesize, rate, modulation, n_frames, seed):
+        import torch
+        import cusat_amd as ca
+        self.ctx = ctx
+        sym, info, cw = ca.gen_frames(framesize, rate, modulation, seed, n_frames, 1000.0, want_cw=True)
+        self.n = n_frames
+        self.sym = torch.from_numpy(sym).cuda()
+        self.cw = torch.from_numpy(cw).cuda()
+        self.info = torch.from_numpy(info).cuda()
+        self.N = ctx.info["n_ldpc"]
+
+    def llr(self, idx, es_db, gen, llr_out, sub=32):
+        """int16 LLRs [len(idx), N] into llr_out; es_db is one value per
+        sub-batch of `sub` frames (one noise level per demap call)."""
+        import torch
+        B = len(idx)
+        for k0 in range(0, B, sub):
+            k1 = min(B, k0 + sub)
+            es = float(es_db[k0 // sub])
+            nv, gain = snr_scales(es)
+            s = self.sym[idx[k0:k1]]
+            noise = torch.randn(s.shape, generator=gen, device=s.device) * float(np.sqrt(nv / 2.0))
+            noisy = (s + noise).contiguous()
+            self.ctx.demap(noisy, llr_out[k0:k1], k1 - k0, nv, gain)
+
+
+def train(framesize=0, rate="2/3", modulation="16APSK", n_iters=20, batch=512, steps=150,
+          es_lo=8.5, es_hi=10.0, pool_frames=2048, seed=20240611, kappa=0.5, lr_alpha=0.01,
+          lr_beta=0.05, log=print):
+    import torch
+    import cusat_amd as ca
+    ctx = ca.FecContext(framesize, rate, modulation, max_trials=50, max_batch=max(batch, 64))
+    q, N = ctx.info["q"], ctx.info["n_ldpc"]
+    pool = FramePool(ctx, framesize, rate, modulation, pool_frames, seed)
+    lc = ca.LearnContext(ctx, batch, n_iters)
+    dev = torch.device("cuda")
+    alpha = torch.ones((n_iters, q), dtype=torch.float32, device=dev)
+    beta = torch.ones((n_iters, q), dtype=torch.float32, device=dev)
+    ma, va, mb, vb = [torch.zeros_like(alpha) for _ in range(4)]
+    ga, gb = torch.zeros_like(alpha), torch.zeros_like(beta)
+    loss = torch.zeros(n_iters, dtype=torch.float32, device=dev)
+    llr16 = torch.empty((batch, N), dtype=torch.int16, device=dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    rng = np.random.default_rng(seed)
+    hist = []
+    t0 = time.time()
+    for step in range(1, steps + 1):
+        idx = torch.from_numpy(rng.choice(pool.n, batch, replace=False)).to(dev)
+        es = rng.uniform(es_lo, es_hi, (batch + 31) // 32)
+        pool.llr(idx, es, gen, llr16)
+        llrf = llr16.float()
+        cw = pool.cw[idx].contiguous()
+        ga.zero_(), gb.zero_(), loss.zero_()
+        lc.fwd_bwd(llrf, cw, batch, alpha, beta, kappa, ga, gb, loss)
+        ca.adam_step(alpha, ga, ma, va, lr_alpha, step, 0.25, 2.0)
+        ca.adam_step(beta, gb, mb, vb, lr_beta, step, 0.0, 16.0)
+        if step == 1 or step % 10 == 0 or step == steps:
+            l = loss.cpu().numpy()
+            hist.append({"step": step, "loss_mean": float(l.mean()), "loss_last": float(l[-1]),
+                         "alpha_mean": float(alpha.mean()), "beta_mean": float(beta.mean()),
+                         "seconds": round(time.time() - t0, 1)})
+            log("train step %d: loss %.5f (last iteration %.5f) alpha %.3f beta %.3f  %.1fs" %
+                (step, l.mean(), l[-1], float(alpha.mean()), float(beta.mean()), time.time() - t0))
+    a = alpha.cpu().numpy()
+    b = beta.cpu().numpy()
+    bq, aq = quantize(a, b)
+    lc.close()
+    return {"framesize": framesize, "rate": rate, "modulation": modulation, "q": q,
+            "n_iters": n_iters, "alpha": a.round(5).tolist(), "beta": b.round(5).tolist(),
+            "alpha_q": aq.tolist(), "beta_q": bq.tolist(),
+            "training": {"batch": batch, "steps": steps, "es_n0_db": [es_lo, es_hi],
+                         "pool_frames": pool_frames, "seed": seed, "kappa": kappa,
+                         "lr_alpha": lr_alpha, "lr_beta": lr_beta, "history": hist}}
+
+
+def evaluate(params_path=DEFAULT_PARAMS, points=(8.8, 9.0, 9.2, 9.4, 9.6), frames=16384, chunk=1024,
+             learned_trials=20, plain_trials=50, pool_frames=2048, seed=777, log=print):
+    """FER and decoded info Gbit/s per Es/N0 of: the plain decoder (T = 50),
+    the learned decoder alone (T = learned_trials) and the learned decoder
+    with plain fallback, on the same noisy LLRs."""
+    import torch
+    import cusat_amd as ca
+    d, bq, aq = load_params(params_path)
+    fs, rate, mod = d["framesize"], d["rate"], d["modulation"]
+    ctx = ca.FecContext(fs, rate, mod, max_trials=plain_trials, max_batch=chunk)
+    ctx.set_params(bq, aq)
+    inf = ctx.info
+    N, K, kb = inf["n_ldpc"], inf["k_ldpc"], inf["k_bch"]
+    pool = FramePool(ctx, fs, rate, mod, pool_frames, seed)
+    dev = torch.device("cuda")
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    rng = np.random.default_rng(seed)
+    llr = torch.empty((chunk, N), dtype=torch.int16, device=dev)
+    hard = torch.empty((chunk, K // 8), dtype=torch.uint8, device=dev)
+    tr = torch.empty(chunk, dtype=torch.int32, device=dev)
+    va = torch.empty(chunk, dtype=torch.int32, device=dev)
+    fb = torch.empty(chunk, dtype=torch.int32, device=dev)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    rows = []
+    for es in points:
+        acc = {m: {"err": 0, "ldpc_fail": 0, "iters": 0, "ms": 0.0, "fallback": 0}
+               for m in ("plain", "learned", "learned+fallback")}
+        for c0 in range(0, frames, chunk):
+            idx = torch.from_numpy(rng.integers(0, pool.n, chunk)).to(dev)
+            pool.llr(idx, np.full((chunk + 31) // 32, es), gen, llr)
+            info_ref = pool.info[idx]
+            for m in acc:
+                torch.cuda.synchronize()
+                ev[0].record()
+                if m == "plain":
+                    ctx.decode(llr, hard, tr, chunk)
+                    ctx.bch_check(hard, va, chunk)
+                elif m == "learned":
+                    ctx.decode_learned(llr, hard, tr, chunk, learned_trials)
+                    ctx.bch_check(hard, va, chunk)
+                else:
+                    ctx.rx_fec_learned(llr, hard, tr, va, fb, chunk, learned_trials)
+                ev[1].record()
+                torch.cuda.synchronize()
+                a = acc[m]
+                a["ms"] += ev[0].elapsed_time(ev[1])
+                ok = (tr >= 0) & (va == 1)
+                good = ok & (hard[:, : kb // 8] == info_ref).all(1)
+                a["err"] += int((~good).sum())
+                a["ldpc_fail"] += int((tr < 0).sum())
+                T = learned_trials if m == "learned" else plain_trials
+                it = torch.where(tr >= 0, T - tr, torch.full_like(tr, T))
+                if m == "learned+fallback":
+                    a["fallback"] += int(fb.sum())
+                    it = torch.where(fb == 1, it + learned_trials, learned_trials - tr)
+                a["iters"] += int(it.sum())
+        for m, a in acc.items():
+            row = {"es_n0_db": es, "decoder": m, "frames": frames, "fer": a["err"] / frames,
+                   "ldpc_fail_rate": a["ldpc_fail"] / frames, "mean_iterations": a["iters"] / frames,
+                   "gbps_info": frames * kb / (a["ms"] * 1e-3) / 1e9, "ms_total": round(a["ms"], 3)}
+            if m == "learned+fallback":
+                row["fallback_rate"] = a["fallback"] / frames
+            rows.append(row)
+            log(json.dumps(row))
+    return rows
+
+
+def fer_crossing(rows, decoder, target=1e-3):
+    """Es/N0 where the FER of `decoder` falls through `target` (log-linear
+    interpolation between the sweep points) and the Gbit/s there."""
+    r = sorted([x for x in rows if x["decoder"] == decoder], key=lambda x: x["es_n0_db"])
+    for a, b in zip(r, r[1:]):
+        if a["fer"] > target >= b["fer"]:
+            fa, fb = np.log10(max(a["fer"], 1e-9)), np.log10(max(b["fer"], 1e-9))
+            w = (fa - np.log10(target)) / (fa - fb) if fa != fb else 0.0
+            return (a["es_n0_db"] + w * (b["es_n0_db"] - a["es_n0_db"]),
+                    a["gbps_info"] + w * (b["gbps_info"] - a["gbps_info"]))
+    return None
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--out", default=DEFAULT_PARAMS)
+    t.add_argument("--steps", type=int, default=150)
+    t.add_argument("--batch", type=int, default=512)
+    t.add_argument("--iters", type=int, default=20)
+    t.add_argument("--pool", type=int, default=2048)
+    e = sub.add_parser("eval")
+    e.add_argument("--params", default=DEFAULT_PARAMS)
+    e.add_argument("--frames", type=int, default=16384)
+    e.add_argument("--points", default="8.8,9.0,9.2,9.4,9.6")
+    e.add_argument("--out", default=None)
+    a = ap.parse_args(argv)
+    log = lambda s: print(s, flush=True)
+    if a.cmd == "train":
+        d = train(n_iters=a.iters, batch=a.batch, steps=a.steps, pool_frames=a.pool, log=log)
+        os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
+        with open(a.out, "w") as f:
+            json.dump(d, f, indent=1)
+        log("wrote " + a.out)
+    else:
+        rows = evaluate(a.params, [float(x) for x in a.points.split(",")], a.frames, log=log)
+        summary = {m: fer_crossing(rows, m) for m in ("plain", "learned", "learned+fallback")}
+        log(json.dumps({"fer_1e-3_crossing": summary}))
+        if a.out:
+            with open(a.out, "w") as f:
+                json.dump({"rows": rows, "fer_1e-3_crossing": summary}, f, indent=1)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -121,6 +121,61 @@ int cusat_rx_fec_batch(cusat_fec_ctx* ctx, const float* sym_dev, int16_t* llr_de
                        int batch, float noise_var, float llr_gain, uint8_t* hard_out,
                        int32_t* trials_out, int32_t* valid_out, void* stream);
 
+/* ---- learned offset / normalised min-sum ----
+ * The reference decoder carries two check-node hooks -- an offset beta inside
+ * finalp_oms_i16 (ldpc_decode_kernel.cu:242-275, g_oms_beta :73) and an
+ * odd-symmetric scale m = (|v| n + 32) >> 6 before the int8 clamp
+ * (nms_scale, :75-90) -- and sets both once, globally (plain: beta = 1, no
+ * scale).  These calls set them per (iteration, layer).
+ *
+ * cusat_ldpc_set_params: beta[t*q + i] >= 0 (LLR units) and alpha[t*q + i]
+ * in 0..1023 (n/64; 0 and 64 leave the message unscaled) for iterations
+ * t < n_iters and layers i < q; iterations t >= n_iters run the plain check
+ * node.  n_iters = 0 clears them.  Not to be called while a decode on this
+ * context is in flight. */
+int cusat_ldpc_set_params(cusat_fec_ctx* ctx, const int16_t* beta, const int16_t* alpha, int n_iters);
+/* cusat_ldpc_decode with the parameters above and max_trials iterations
+ * (replaces decode_launch_async with the hooks set). */
+int cusat_ldpc_decode_learned(cusat_fec_ctx* ctx, const int16_t* llr_dev, uint8_t* hard_out,
+                              int32_t* trials_out, int batch, int max_trials, void* stream);
+/* Learned decode (learned_trials iterations) -> BCH check -> every frame
+ * that fails rx_fec_chain::frame_passes_fec_checks (rx_fec_chain.h:193-196)
+ * is decoded again by the plain decoder (the context's max_trials) and
+ * BCH-checked again, all on one stream with no host round trip.
+ * fallback_out (B ints, or NULL) = 1 for the frames that were redone; their
+ * trials_out / valid_out / hard_out come from the plain decoder. */
+int cusat_rx_fec_learned(cusat_fec_ctx* ctx, const int16_t* llr_dev, uint8_t* hard_out,
+                         int32_t* trials_out, int32_t* valid_out, int32_t* fallback_out, int batch,
+                         int learned_trials, void* stream);
+
+/* ---- training of the learned parameters (fp32) ----
+ * No reference counterpart (the reference never trains its hooks).  A
+ * training context holds, for up to max_batch frames of the decoder
+ * context's code and n_iters unrolled iterations, the forward state the
+ * backward pass needs.  The fp32 model is the decoder above without
+ * integer rounding: x = L - m_old; y = clamp(alpha * sign * max(min_other|x|
+ * - beta, 0), -127, 127); L += y - m_old, step-synchronous over the
+ * context's schedule; loss = mean over iterations t < n_iters, frames and
+ * bits of softplus(-kappa * s * L_t), s = +1 for a 0 bit. */
+typedef struct cusat_learn_ctx cusat_learn_ctx;
+int cusat_learn_ctx_create(cusat_learn_ctx** out, const cusat_fec_ctx* fec, int max_batch, int n_iters);
+int cusat_learn_ctx_destroy(cusat_learn_ctx* lc);
+/* One forward + backward pass.  llr_dev: float [B][n_ldpc]; cw_dev: the
+ * transmitted codeword bits, [B][n_ldpc/8] MSB first; alpha_dev, beta_dev:
+ * float [n_iters][q].  Accumulates (+=) into grad_alpha_dev / grad_beta_dev
+ * (float [n_iters][q]) and loss_dev (float [n_iters], per-iteration mean
+ * BCE).  app_out_dev (float [B][n_ldpc], final APP) and grad_llr_dev
+ * (float [B][n_ldpc], d loss / d llr) may be NULL. */
+int cusat_learn_fwd_bwd(cusat_learn_ctx* lc, const float* llr_dev, const uint8_t* cw_dev, int batch,
+                        const float* alpha_dev, const float* beta_dev, float kappa,
+                        float* grad_alpha_dev, float* grad_beta_dev, float* loss_dev,
+                        float* app_out_dev, float* grad_llr_dev, void* stream);
+/* Adam step on n device floats (bias-corrected, step >= 1), then clamp to
+ * [lo, hi]. */
+int cusat_learn_adam(float* param_dev, const float* grad_dev, float* m_dev, float* v_dev, int n,
+                     float lr, float beta1, float beta2, float eps, int step, float lo, float hi,
+                     void* stream);
+
 /* Host-only (no device): the constellation [2 << n_mod floats] and
  * deinterleaver column offsets [8] of a MODCOD, its bits per symbol and
  * interleaver rows (0 for QPSK).  Mirrors make_constellation /
