@@ -84,7 +84,7 @@ def lib():
         L.cusat_rx_fec_learned.argtypes = [vp, vp, vp, vp, vp, vp, i, i, vp]
         L.cusat_learn_ctx_create.argtypes = [ctypes.POINTER(vp), vp, i, i]
         L.cusat_learn_ctx_destroy.argtypes = [vp]
-        L.cusat_learn_fwd_bwd.argtypes = [vp, vp, vp, i, vp, vp, f, vp, vp, vp, vp, vp, vp]
+        L.cusat_learn_fwd_bwd.argtypes = [vp, vp, vp, i, vp, vp, f, u, vp, vp, vp, vp, vp, vp]
         L.cusat_learn_adam.argtypes = [vp, vp, vp, vp, i, f, f, f, f, i, f, f, vp]
         _lib = L
     return _lib
@@ -185,6 +185,9 @@ class FecContext:
                "cusat_rx_fec_learned")
 
 
+LEARN_QUANTIZED = 0x1
+
+
 class LearnContext:
     """fp32 forward/backward state for training the learned parameters of
     one code over n_iters unrolled iterations (cusat_learn_* of the C ABI)."""
@@ -204,9 +207,10 @@ class LearnContext:
     __del__ = close
 
     def fwd_bwd(self, llr, cw, batch, alpha, beta, kappa, grad_alpha, grad_beta, loss,
-                app_out=None, grad_llr=None, stream=None):
+                app_out=None, grad_llr=None, quantized=False, stream=None):
         _check(lib().cusat_learn_fwd_bwd(self._h, _ptr(llr), _ptr(cw), batch, _ptr(alpha), _ptr(beta),
-                                         kappa, _ptr(grad_alpha), _ptr(grad_beta), _ptr(loss),
+                                         kappa, LEARN_QUANTIZED if quantized else 0,
+                                         _ptr(grad_alpha), _ptr(grad_beta), _ptr(loss),
                                          _ptr(app_out), _ptr(grad_llr), _stream(stream)),
                "cusat_learn_fwd_bwd")
 

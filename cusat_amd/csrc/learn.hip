@@ -53,6 +53,7 @@ struct largs {
     const int* lay_moff;
     const int* steps;  // [n_steps][2], -1 = idle lane
     int N, K, q, E, T;
+    int quant;  // 1: the integer decoder's check node (straight-through gradients)
     float* L;   // [B][N] APP
     float* M;   // [B][E] messages
     float* X;   // [B][T][E] read-phase values
@@ -120,6 +121,36 @@ __device__ __forceinline__ void min_sum(const float* x, const bool* ok, float& m
     }
 }
 
+// Check-node output y of one edge (mu = min of |x| over the other edges,
+// sg = its sign) and what its derivatives need: r (the offset magnitude),
+// act (the offset is active: mu above it) and pass (the +-127 clamp passes).
+//  * fp32 model: y = clamp(al sg max(mu - be, 0), -127, 127).
+//  * quantised model = the integer decoder: beta_q = max(rint(be), 0),
+//    n = clamp(rint(64 al), 1, 1023), |y| = min(floor((r n + 32) / 64), 127),
+//    r = max(mu - beta_q, 0) (ldpc_decode_kernel.cu:75-90,242-275); with
+//    integer inputs every value is an exact fp32 integer (r n < 2^24).
+//    Gradients go straight through the rounding: dy/dal = sg r,
+//    dy/dbe = -al sg, dy/dmu = al sg, gated by act and pass.
+__device__ __forceinline__ void cn_edge(float mu, float sg, float al, float be, int quant, float& y,
+                                        float& r, bool& act, bool& pass)
+{
+    if (quant) {
+        const float bq = fmaxf(rintf(be), 0.0f);
+        const float n = fminf(fmaxf(rintf(64.0f * al), 1.0f), 1023.0f);
+        r = fmaxf(mu - bq, 0.0f);
+        const float m = floorf((r * n + 32.0f) * (1.0f / 64.0f));
+        pass = m <= 127.0f;
+        y = sg * fminf(m, 127.0f);
+        act = mu > bq;
+    } else {
+        r = fmaxf(mu - be, 0.0f);
+        const float v = al * sg * r;
+        pass = fabsf(v) <= 127.0f;
+        y = fminf(fmaxf(v, -127.0f), 127.0f);
+        act = mu > be;
+    }
+}
+
 __device__ __forceinline__ float wave_sum(float v)
 {
 #pragma unroll
@@ -162,7 +193,9 @@ __global__ void __launch_bounds__(BLOCK) fwd_step(largs a, int t, int s)
         for (int e = 0; e < CM + 2; e++) {
             const float sg = (par ^ (x[e] < 0.0f ? 1u : 0u)) ? -1.0f : 1.0f;
             const float mu = e == i1 ? mn2 : mn1;
-            y[e] = fminf(fmaxf(al * sg * fmaxf(mu - be, 0.0f), -127.0f), 127.0f);
+            float r;
+            bool act_e, pass;
+            cn_edge(mu, sg, al, be, a.quant, y[e], r, act_e, pass);
         }
     }
     __syncthreads();
@@ -212,10 +245,12 @@ __global__ void __launch_bounds__(BLOCK) bwd_step(largs a, int t, int s)
             const float gy = gl[e] + gM[mi[e]];
             const float sg = (par ^ (x[e] < 0.0f ? 1u : 0u)) ? -1.0f : 1.0f;
             const float mu = e == i1 ? mn2 : mn1;
-            const float r = fmaxf(mu - be, 0.0f);
-            if (fabsf(al * sg * r) <= 127.0f) {  // clamp passes the gradient on [-127, 127]
+            float y, r;
+            bool act_e, pass;
+            cn_edge(mu, sg, al, be, a.quant, y, r, act_e, pass);
+            if (pass) {  // the clamp passes the gradient on [-127, 127]
                 gal += gy * sg * r;
-                if (mu > be) {
+                if (act_e) {
                     gbe -= gy * al * sg;
                     dmu[e] = gy * al * sg;
                 }
@@ -425,7 +460,8 @@ extern "C" int cusat_learn_ctx_destroy(cusat_learn_ctx* lc)
 }
 
 extern "C" int cusat_learn_fwd_bwd(cusat_learn_ctx* lc, const float* llr, const uint8_t* cw, int batch,
-                                   const float* alpha, const float* beta, float kappa, float* grad_alpha,
+                                   const float* alpha, const float* beta, float kappa, unsigned flags,
+                                   float* grad_alpha,
                                    float* grad_beta, float* loss, float* app_out, float* grad_llr, void* stream)
 {
     if (!lc || !llr || !cw || !alpha || !beta || !grad_alpha || !grad_beta || !loss) {
@@ -449,6 +485,7 @@ extern "C" int cusat_learn_fwd_bwd(cusat_learn_ctx* lc, const float* llr, const 
     a.q = lc->q;
     a.E = lc->E;
     a.T = lc->T;
+    a.quant = (flags & CUSAT_LEARN_QUANTIZED) ? 1 : 0;
     a.L = lc->L;
     a.M = lc->M;
     a.X = lc->X;

@@ -79,7 +79,7 @@ class FramePool:
 
 def train(framesize=0, rate="2/3", modulation="16APSK", n_iters=20, batch=512, steps=150,
           es_lo=8.5, es_hi=10.0, pool_frames=2048, seed=20240611, kappa=0.5, lr_alpha=0.01,
-          lr_beta=0.05, log=print):
+          lr_beta=0.05, quantized=True, log=print):
     import torch
     import cusat_amd as ca
     ctx = ca.FecContext(framesize, rate, modulation, max_trials=50, max_batch=max(batch, 64))
@@ -105,7 +105,7 @@ def train(framesize=0, rate="2/3", modulation="16APSK", n_iters=20, batch=512, s
         llrf = llr16.float()
         cw = pool.cw[idx].contiguous()
         ga.zero_(), gb.zero_(), loss.zero_()
-        lc.fwd_bwd(llrf, cw, batch, alpha, beta, kappa, ga, gb, loss)
+        lc.fwd_bwd(llrf, cw, batch, alpha, beta, kappa, ga, gb, loss, quantized=quantized)
         ca.adam_step(alpha, ga, ma, va, lr_alpha, step, 0.25, 2.0)
         ca.adam_step(beta, gb, mb, vb, lr_beta, step, 0.0, 16.0)
         if step == 1 or step % 10 == 0 or step == steps:
@@ -124,7 +124,8 @@ def train(framesize=0, rate="2/3", modulation="16APSK", n_iters=20, batch=512, s
             "alpha_q": aq.tolist(), "beta_q": bq.tolist(),
             "training": {"batch": batch, "steps": steps, "es_n0_db": [es_lo, es_hi],
                          "pool_frames": pool_frames, "seed": seed, "kappa": kappa,
-                         "lr_alpha": lr_alpha, "lr_beta": lr_beta, "history": hist}}
+                         "lr_alpha": lr_alpha, "lr_beta": lr_beta, "quantized": quantized,
+                         "history": hist}}
 
 
 def evaluate(params_path=DEFAULT_PARAMS, points=(8.8, 9.0, 9.2, 9.4, 9.6), frames=16384, chunk=1024,
@@ -138,6 +139,10 @@ def evaluate(params_path=DEFAULT_PARAMS, points=(8.8, 9.0, 9.2, 9.4, 9.6), frame
     fs, rate, mod = d["framesize"], d["rate"], d["modulation"]
     ctx = ca.FecContext(fs, rate, mod, max_trials=plain_trials, max_batch=chunk)
     ctx.set_params(bq, aq)
+    # the same structure (learned_trials + plain fallback) with the plain
+    # check node, to separate the parameters' gain from the early cut-off's
+    ctx_id = ca.FecContext(fs, rate, mod, max_trials=plain_trials, max_batch=chunk)
+    ctx_id.set_params(np.ones_like(bq), np.full_like(aq, 64))
     inf = ctx.info
     N, K, kb = inf["n_ldpc"], inf["k_ldpc"], inf["k_bch"]
     pool = FramePool(ctx, fs, rate, mod, pool_frames, seed)
@@ -154,7 +159,7 @@ def evaluate(params_path=DEFAULT_PARAMS, points=(8.8, 9.0, 9.2, 9.4, 9.6), frame
     rows = []
     for es in points:
         acc = {m: {"err": 0, "ldpc_fail": 0, "iters": 0, "ms": 0.0, "fallback": 0}
-               for m in ("plain", "learned", "learned+fallback")}
+               for m in ("plain", "learned", "learned+fallback", "identity+fallback")}
         for c0 in range(0, frames, chunk):
             idx = torch.from_numpy(rng.integers(0, pool.n, chunk)).to(dev)
             pool.llr(idx, np.full((chunk + 31) // 32, es), gen, llr)
@@ -168,8 +173,10 @@ def evaluate(params_path=DEFAULT_PARAMS, points=(8.8, 9.0, 9.2, 9.4, 9.6), frame
                 elif m == "learned":
                     ctx.decode_learned(llr, hard, tr, chunk, learned_trials)
                     ctx.bch_check(hard, va, chunk)
-                else:
+                elif m == "learned+fallback":
                     ctx.rx_fec_learned(llr, hard, tr, va, fb, chunk, learned_trials)
+                else:
+                    ctx_id.rx_fec_learned(llr, hard, tr, va, fb, chunk, learned_trials)
                 ev[1].record()
                 torch.cuda.synchronize()
                 a = acc[m]
@@ -180,7 +187,7 @@ def evaluate(params_path=DEFAULT_PARAMS, points=(8.8, 9.0, 9.2, 9.4, 9.6), frame
                 a["ldpc_fail"] += int((tr < 0).sum())
                 T = learned_trials if m == "learned" else plain_trials
                 it = torch.where(tr >= 0, T - tr, torch.full_like(tr, T))
-                if m == "learned+fallback":
+                if m.endswith("+fallback"):
                     a["fallback"] += int(fb.sum())
                     it = torch.where(fb == 1, it + learned_trials, learned_trials - tr)
                 a["iters"] += int(it.sum())
@@ -188,7 +195,7 @@ def evaluate(params_path=DEFAULT_PARAMS, points=(8.8, 9.0, 9.2, 9.4, 9.6), frame
             row = {"es_n0_db": es, "decoder": m, "frames": frames, "fer": a["err"] / frames,
                    "ldpc_fail_rate": a["ldpc_fail"] / frames, "mean_iterations": a["iters"] / frames,
                    "gbps_info": frames * kb / (a["ms"] * 1e-3) / 1e9, "ms_total": round(a["ms"], 3)}
-            if m == "learned+fallback":
+            if m.endswith("+fallback"):
                 row["fallback_rate"] = a["fallback"] / frames
             rows.append(row)
             log(json.dumps(row))
@@ -217,6 +224,10 @@ def main(argv=None):
     t.add_argument("--batch", type=int, default=512)
     t.add_argument("--iters", type=int, default=20)
     t.add_argument("--pool", type=int, default=2048)
+    t.add_argument("--lr-alpha", type=float, default=0.01)
+    t.add_argument("--lr-beta", type=float, default=0.05)
+    t.add_argument("--float-model", action="store_true", help="train the fp32 model without rounding")
+    t.add_argument("--es", default="8.5,10.0", help="training Es/N0 range in dB")
     e = sub.add_parser("eval")
     e.add_argument("--params", default=DEFAULT_PARAMS)
     e.add_argument("--frames", type=int, default=16384)
@@ -225,14 +236,17 @@ def main(argv=None):
     a = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)
     if a.cmd == "train":
-        d = train(n_iters=a.iters, batch=a.batch, steps=a.steps, pool_frames=a.pool, log=log)
+        lo, hi = [float(x) for x in a.es.split(",")]
+        d = train(n_iters=a.iters, batch=a.batch, steps=a.steps, pool_frames=a.pool, es_lo=lo, es_hi=hi,
+                  lr_alpha=a.lr_alpha, lr_beta=a.lr_beta, quantized=not a.float_model, log=log)
         os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
         with open(a.out, "w") as f:
             json.dump(d, f, indent=1)
         log("wrote " + a.out)
     else:
         rows = evaluate(a.params, [float(x) for x in a.points.split(",")], a.frames, log=log)
-        summary = {m: fer_crossing(rows, m) for m in ("plain", "learned", "learned+fallback")}
+        summary = {m: fer_crossing(rows, m)
+                   for m in ("plain", "learned", "learned+fallback", "identity+fallback")}
         log(json.dumps({"fer_1e-3_crossing": summary}))
         if a.out:
             with open(a.out, "w") as f:

@@ -160,6 +160,12 @@ int cusat_rx_fec_learned(cusat_fec_ctx* ctx, const int16_t* llr_dev, uint8_t* ha
 typedef struct cusat_learn_ctx cusat_learn_ctx;
 int cusat_learn_ctx_create(cusat_learn_ctx** out, const cusat_fec_ctx* fec, int max_batch, int n_iters);
 int cusat_learn_ctx_destroy(cusat_learn_ctx* lc);
+/* flags of cusat_learn_fwd_bwd: the forward pass is the integer decoder's
+ * check node (beta rounded, alpha as round(64 alpha)/64 with the
+ * (|v| n + 32) >> 6 rounding, +-127 clamp), gradients straight through the
+ * rounding; with integer LLRs the forward values are the integer decoder's
+ * (without early exit and the int16 rails). */
+#define CUSAT_LEARN_QUANTIZED 0x1u
 /* One forward + backward pass.  llr_dev: float [B][n_ldpc]; cw_dev: the
  * transmitted codeword bits, [B][n_ldpc/8] MSB first; alpha_dev, beta_dev:
  * float [n_iters][q].  Accumulates (+=) into grad_alpha_dev / grad_beta_dev
@@ -167,7 +173,7 @@ int cusat_learn_ctx_destroy(cusat_learn_ctx* lc);
  * BCE).  app_out_dev (float [B][n_ldpc], final APP) and grad_llr_dev
  * (float [B][n_ldpc], d loss / d llr) may be NULL. */
 int cusat_learn_fwd_bwd(cusat_learn_ctx* lc, const float* llr_dev, const uint8_t* cw_dev, int batch,
-                        const float* alpha_dev, const float* beta_dev, float kappa,
+                        const float* alpha_dev, const float* beta_dev, float kappa, unsigned flags,
                         float* grad_alpha_dev, float* grad_beta_dev, float* loss_dev,
                         float* app_out_dev, float* grad_llr_dev, void* stream);
 /* Adam step on n device floats (bias-corrected, step >= 1), then clamp to

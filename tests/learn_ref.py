@@ -71,9 +71,15 @@ class TorchLayered:
         t = lambda a, dt: torch.as_tensor(np.concatenate(a), dtype=dt, device=device)
         return t(V, torch.long), t(MI, torch.long), t(MASK, torch.bool), t(LAY, torch.long)
 
-    def run(self, llr, bits, alpha, beta, kappa):
+    def run(self, llr, bits, alpha, beta, kappa, quant=False):
         """llr [B, N], bits [B, N] (0/1), alpha/beta [T, q] (tensors, may
-        require grad).  Returns (final APP, per-iteration losses, loss)."""
+        require grad).  Returns (final APP, per-iteration losses, loss).
+
+        quant: the integer decoder's check node -- beta_q = max(round(beta), 0),
+        n = clamp(round(64 alpha), 1, 1023), |y| = min(floor((r n + 32) / 64),
+        127), r = relu(mu - beta_q) -- with straight-through gradients
+        dy/dalpha = s r, dy/dbeta = -alpha s [mu > beta_q], dy/dmu = alpha s
+        [mu > beta_q], all gated by floor(...) <= 127."""
         B = llr.shape[0]
         T = alpha.shape[0]
         dev, dt = llr.device, llr.dtype
@@ -98,7 +104,18 @@ class TorchLayered:
                 sg = torch.where(((par + neg.long()) % 2) == 1, -1.0, 1.0).to(dt)
                 al = alpha[t][LAY].view(1, -1, 1)
                 be = beta[t][LAY].view(1, -1, 1)
-                y = torch.clamp(al * sg * torch.relu(mu - be), -127.0, 127.0)
+                if quant:
+                    bq = torch.clamp(torch.round(be.detach()), min=0.0)
+                    n = torch.clamp(torch.round(64.0 * al.detach()), 1.0, 1023.0)
+                    r = torch.relu(mu - bq)
+                    m = torch.floor((r * n + 32.0) / 64.0)
+                    pas = (m <= 127.0).to(dt)
+                    act = (mu > bq).to(dt)
+                    yh = sg * torch.clamp(m, max=127.0)
+                    f = pas * sg * (al * r.detach() + al.detach() * act * (mu - be))
+                    y = yh.detach() + (f - f.detach())
+                else:
+                    y = torch.clamp(al * sg * torch.relu(mu - be), -127.0, 127.0)
                 y = torch.where(MASK, y, torch.zeros((), dtype=dt, device=dev))
                 fm = MASK.reshape(-1)
                 delta = (y - torch.where(MASK, mold, torch.zeros((), dtype=dt, device=dev))).reshape(B, -1)

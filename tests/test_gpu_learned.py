@@ -112,8 +112,11 @@ def test_rx_fec_learned_fallback_vs_oracle():
     np.testing.assert_array_equal(va.cpu().numpy(), np.where(redo, v_p, v_l))
 
 
-@pytest.mark.parametrize("name,fs,rate,B,T", [("s2_c4", 1, "1/2", 3, 3), ("s2_b6", 0, "2/3", 1, 2)])
-def test_training_kernels_vs_torch_reference(name, fs, rate, B, T):
+@pytest.mark.parametrize("name,fs,rate,B,T,quant", [("s2_c4", 1, "1/2", 3, 3, False),
+                                                    ("s2_b6", 0, "2/3", 1, 2, False),
+                                                    ("s2_c4", 1, "1/2", 3, 4, True),
+                                                    ("s2_b6", 0, "2/3", 2, 3, True)])
+def test_training_kernels_vs_torch_reference(name, fs, rate, B, T, quant):
     torch = _torch()
     from tests.learn_ref import TorchLayered
     ctx = ca.FecContext(fs, rate, "QPSK", max_trials=50, max_batch=B)
@@ -131,13 +134,19 @@ def test_training_kernels_vs_torch_reference(name, fs, rate, B, T):
     llr = np.rint(4 * ((1 - 2 * bits.astype(np.float64)) * 3.0 + rng.normal(0, 3.5, bits.shape))) / 4
     alpha = rng.choice([1.0, 1.0, 2.0], (T, q))
     beta = rng.integers(0, 7, (T, q)) / 4.0
+    if quant:
+        # the integer decoder's check node: integer LLRs, any alpha / beta
+        # (rounded inside), every forward value an exact integer
+        llr = np.rint((1 - 2 * bits.astype(np.float64)) * 9.0 + rng.normal(0, 8.0, bits.shape))
+        alpha = rng.uniform(0.7, 1.3, (T, q))
+        beta = rng.uniform(0.0, 2.6, (T, q))
     kappa = 0.5
 
     ref = TorchLayered(name)
     a_t = torch.tensor(alpha, requires_grad=True)
     b_t = torch.tensor(beta, requires_grad=True)
     x_t = torch.tensor(llr, requires_grad=True)
-    L_r, losses_r, loss_r = ref.run(x_t, torch.from_numpy(bits), a_t, b_t, kappa)
+    L_r, losses_r, loss_r = ref.run(x_t, torch.from_numpy(bits), a_t, b_t, kappa, quant=quant)
     ga_r, gb_r, gx_r = torch.autograd.grad(loss_r, (a_t, b_t, x_t))
 
     lc = ca.LearnContext(ctx, B, T)
@@ -148,7 +157,7 @@ def test_training_kernels_vs_torch_reference(name, fs, rate, B, T):
     loss = torch.zeros(T, dtype=torch.float32, device="cuda")
     app = torch.zeros_like(llr_d)
     gx = torch.zeros_like(llr_d)
-    lc.fwd_bwd(llr_d, cw_d, B, al_d, be_d, kappa, ga, gb, loss, app, gx)
+    lc.fwd_bwd(llr_d, cw_d, B, al_d, be_d, kappa, ga, gb, loss, app, gx, quantized=quant)
     torch.cuda.synchronize()
 
     # forward: exact (see above)

@@ -52,6 +52,24 @@ def test_torch_reference_is_the_plain_decoder(name, fs, rate, B, T):
     assert not np.array_equal(_pack((L2[:, : ref.K] < 0).numpy()), h_o)
 
 
+@pytest.mark.parametrize("name,fs,rate,B,T", [("s2_c4", 1, "1/2", 4, 4), ("s2_b6", 0, "2/3", 2, 3)])
+def test_quantised_torch_reference_is_the_integer_decoder(name, fs, rate, B, T):
+    """The quantised model with real-valued (alpha, beta) is the integer
+    decoder with the rounded parameters of learn.quantize."""
+    llr, bits, _ = _noisy_llr(fs, rate, B, 21, 3.0, 4.0)
+    ref = TorchLayered(name)
+    rng = np.random.default_rng(2)
+    alpha = rng.uniform(0.6, 1.4, (T, ref.q))
+    beta = rng.uniform(0.0, 2.7, (T, ref.q))
+    L, _, _ = ref.run(torch.from_numpy(llr.astype(np.float64)), torch.from_numpy(bits),
+                      torch.from_numpy(alpha), torch.from_numpy(beta), 0.5, quant=True)
+    bq, aq = learn.quantize(alpha, beta)
+    code = oc.Code(name)
+    h_o, r_o = code.decode(llr, T, beta=bq, alpha=aq, n_param_iters=T)
+    assert np.all(r_o == -1)
+    np.testing.assert_array_equal(_pack((L[:, : ref.K] < 0).numpy()), h_o)
+
+
 def test_torch_reference_gradients_vs_central_differences():
     llr, bits, _ = _noisy_llr(1, "1/2", 1, 3, 2.5, 3.0, integer=False)
     ref = TorchLayered("s2_c4")
