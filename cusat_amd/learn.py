@@ -77,9 +77,9 @@ class FramePool:
             self.ctx.demap(noisy, llr_out[k0:k1], k1 - k0, nv, gain)
 
 
-def train(framesize=0, rate="2/3", modulation="16APSK", n_iters=20, batch=512, steps=150,
-          es_lo=8.5, es_hi=10.0, pool_frames=2048, seed=20240611, kappa=0.5, lr_alpha=0.01,
-          lr_beta=0.05, quantized=True, log=print):
+def train(framesize=0, rate="2/3", modulation="16APSK", n_iters=20, batch=512, steps=400,
+          es_lo=8.9, es_hi=9.6, pool_frames=2048, seed=20240611, kappa=0.5, lr_alpha=0.005,
+          lr_beta=0.02, quantized=True, log=print):
     import torch
     import cusat_amd as ca
     ctx = ca.FecContext(framesize, rate, modulation, max_trials=50, max_batch=max(batch, 64))
@@ -220,14 +220,14 @@ def main(argv=None):
     sub = ap.add_subparsers(dest="cmd", required=True)
     t = sub.add_parser("train")
     t.add_argument("--out", default=DEFAULT_PARAMS)
-    t.add_argument("--steps", type=int, default=150)
+    t.add_argument("--steps", type=int, default=400)
     t.add_argument("--batch", type=int, default=512)
     t.add_argument("--iters", type=int, default=20)
     t.add_argument("--pool", type=int, default=2048)
-    t.add_argument("--lr-alpha", type=float, default=0.01)
-    t.add_argument("--lr-beta", type=float, default=0.05)
+    t.add_argument("--lr-alpha", type=float, default=0.005)
+    t.add_argument("--lr-beta", type=float, default=0.02)
     t.add_argument("--float-model", action="store_true", help="train the fp32 model without rounding")
-    t.add_argument("--es", default="8.5,10.0", help="training Es/N0 range in dB")
+    t.add_argument("--es", default="8.9,9.6", help="training Es/N0 range in dB")
     e = sub.add_parser("eval")
     e.add_argument("--params", default=DEFAULT_PARAMS)
     e.add_argument("--frames", type=int, default=16384)
