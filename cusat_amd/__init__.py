@@ -86,6 +86,13 @@ def lib():
         L.cusat_learn_ctx_destroy.argtypes = [vp]
         L.cusat_learn_fwd_bwd.argtypes = [vp, vp, vp, i, vp, vp, f, u, vp, vp, vp, vp, vp, vp]
         L.cusat_learn_adam.argtypes = [vp, vp, vp, vp, i, f, f, f, f, i, f, f, vp]
+        L.cusat_stream_create.argtypes = [ctypes.POINTER(vp), vp, i, i, u]
+        L.cusat_stream_destroy.argtypes = [vp]
+        L.cusat_stream_plan.argtypes = [vp, vp, i, vp]
+        L.cusat_stream_get_tuning.argtypes = [vp, ctypes.POINTER(StreamTuning), ctypes.POINTER(i)]
+        L.cusat_stream_decode.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.cusat_stream_autotune.argtypes = [vp, vp, vp, vp, vp, vp, i, vp, ctypes.POINTER(StreamTuning),
+                                            ctypes.POINTER(f)]
         _lib = L
     return _lib
 
@@ -186,6 +193,65 @@ class FecContext:
 
 
 LEARN_QUANTIZED = 0x1
+STREAM_LEARNED = 0x1
+STREAM_ORDER_GIVEN, STREAM_ORDER_COST = 0, 1
+STREAM_FRAME = np.dtype([("code", np.int32), ("reserved", np.int32), ("llr_offset", np.int64),
+                         ("hard_offset", np.int64)])
+
+
+class StreamTuning(ctypes.Structure):
+    _fields_ = [("wg_per_cu", ctypes.c_int), ("frames_per_job", ctypes.c_int), ("order", ctypes.c_int),
+                ("reserved", ctypes.c_int)]
+
+    def as_dict(self):
+        return {"wg_per_cu": self.wg_per_cu, "frames_per_job": self.frames_per_job, "order": self.order}
+
+
+class StreamContext:
+    """Persistent mixed-stream decoder over several FecContexts
+    (cusat_stream_* of the C ABI).  frames: STREAM_FRAME records."""
+
+    def __init__(self, contexts, max_frames, flags=0):
+        self.contexts = list(contexts)  # keep them alive
+        arr = (ctypes.c_void_p * len(self.contexts))(*[c._h.value for c in self.contexts])
+        h = ctypes.c_void_p()
+        _check(lib().cusat_stream_create(ctypes.byref(h), ctypes.cast(arr, ctypes.c_void_p), len(arr),
+                                         max_frames, flags), "cusat_stream_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cusat_stream_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def plan(self, frames, tuning=None):
+        frames = np.ascontiguousarray(frames, STREAM_FRAME)
+        t = None
+        if tuning is not None:
+            t = StreamTuning(tuning["wg_per_cu"], tuning["frames_per_job"], tuning["order"], 0)
+        _check(lib().cusat_stream_plan(self._h, _ptr(frames), len(frames),
+                                       ctypes.byref(t) if t is not None else None), "cusat_stream_plan")
+
+    def tuning(self):
+        t, m = StreamTuning(), ctypes.c_int()
+        _check(lib().cusat_stream_get_tuning(self._h, ctypes.byref(t), ctypes.byref(m)),
+               "cusat_stream_get_tuning")
+        d = t.as_dict()
+        d["max_wg_per_cu"] = m.value
+        return d
+
+    def decode(self, llr, hard, trials, valid, fallback=None, stream=None):
+        _check(lib().cusat_stream_decode(self._h, _ptr(llr), _ptr(hard), _ptr(trials), _ptr(valid),
+                                         _ptr(fallback), _stream(stream)), "cusat_stream_decode")
+
+    def autotune(self, llr, hard, trials, valid, fallback=None, reps=3, stream=None):
+        t, ms = StreamTuning(), ctypes.c_float()
+        _check(lib().cusat_stream_autotune(self._h, _ptr(llr), _ptr(hard), _ptr(trials), _ptr(valid),
+                                           _ptr(fallback), reps, _stream(stream), ctypes.byref(t),
+                                           ctypes.byref(ms)), "cusat_stream_autotune")
+        return t.as_dict(), ms.value
 
 
 class LearnContext:

@@ -143,18 +143,22 @@ struct kparams {
 // VN group's first pair dword, number of slots, the slots' dword indices.
 constexpr int GROUP_WORDS = 8;
 
+// Decode one frame with the whole workgroup (LANES x 192 threads; a
+// workgroup of 384 threads runs a LANES = 1 code with its second lane idle,
+// since every step of a one-lane schedule has lane 1 = -1).  llr: the frame's
+// N LLRs; rec_base: its message records; h: its K/8 hard bytes.  Returns
+// T - t on convergence at trial t, else -1.  Every thread of the workgroup
+// must call it (it contains barriers); it starts and ends with the LDS free.
 template <int CMAX, bool EXACT, int LANES>
-__global__ void __launch_bounds__(LANES* LANE_THREADS)
-    ldpc_kernel(const int16_t* __restrict__ llr_all, uint4* __restrict__ msg_all,
-                uint8_t* __restrict__ hard_all, int32_t* __restrict__ result,
-                const circ_rec* __restrict__ circ, const int* __restrict__ lay_cnt,
-                const int* __restrict__ lay_off, const int* __restrict__ steps,
-                const int* __restrict__ grp_off, const int* __restrict__ groups,
-                const int* __restrict__ beta, const int* __restrict__ alpha,
-                const int32_t* __restrict__ active, kparams p)
+__device__ __forceinline__ int decode_frame(unsigned char* lds_raw, const int16_t* __restrict__ llr,
+                                            uint4* __restrict__ rec_base, uint8_t* __restrict__ h,
+                                            const circ_rec* __restrict__ circ,
+                                            const int* __restrict__ lay_cnt, const int* __restrict__ lay_off,
+                                            const int* __restrict__ steps, const int* __restrict__ grp_off,
+                                            const int* __restrict__ groups, const int* __restrict__ beta,
+                                            const int* __restrict__ alpha, const kparams& p)
 {
     using R = rec_t<CMAX>;
-    extern __shared__ __align__(16) unsigned char lds_raw[];
     uint32_t* app32 = reinterpret_cast<uint32_t*>(lds_raw);  // K/2 dwords
     uint32_t* par32 = app32 + p.K / 2;                        // R/2 dwords
     uint32_t* dsl32 = par32 + p.R / 2;                        // n_slots * 180 dwords
@@ -167,18 +171,12 @@ __global__ void __launch_bounds__(LANES* LANE_THREADS)
     int16_t* par16 = reinterpret_cast<int16_t*>(par32);
 
     constexpr int NTHR = LANES * LANE_THREADS;
-    const int cw = blockIdx.x;
-    // frames masked off (active[cw] == 0) keep their outputs; the whole
-    // workgroup leaves before its first barrier
-    if (active && active[cw] == 0) return;
     const int t = threadIdx.x;
     const int lane = LANES == 2 ? __builtin_amdgcn_readfirstlane(t >= LANE_THREADS ? 1 : 0) : 0;
     const int tt = t - lane * LANE_THREADS;
     const int tt4 = 4 * tt;
     const bool act = tt < HALF;
     const int q = p.q;
-    const int16_t* llr = llr_all + (size_t)cw * p.N;
-    uint4* rec_base = msg_all + (size_t)cw * q * HALF * R::V4;
 
     for (int k = t; k < 2 * p.n_steps; k += NTHR) sched_l[k] = steps[k];
     for (int k = t; k <= p.n_steps; k += NTHR) goff_l[k] = grp_off[k];
@@ -435,14 +433,34 @@ __global__ void __launch_bounds__(LANES* LANE_THREADS)
     }
 
     // hard decisions, K bits MSB first (ldpc_decode_kernel.cu:468-493)
-    uint8_t* h = hard_all + (size_t)cw * (p.K >> 3);
     for (int b = t; b < (p.K >> 3); b += NTHR) {
         int byte = 0;
 #pragma unroll
         for (int k = 0; k < 8; k++) byte |= (app16[vn_index(8 * b + k)] < 0 ? 1 : 0) << (7 - k);
         h[b] = (uint8_t)byte;
     }
-    if (t == 0) result[cw] = res;
+    return res;
+}
+
+template <int CMAX, bool EXACT, int LANES>
+__global__ void __launch_bounds__(LANES* LANE_THREADS)
+    ldpc_kernel(const int16_t* __restrict__ llr_all, uint4* __restrict__ msg_all,
+                uint8_t* __restrict__ hard_all, int32_t* __restrict__ result,
+                const circ_rec* __restrict__ circ, const int* __restrict__ lay_cnt,
+                const int* __restrict__ lay_off, const int* __restrict__ steps,
+                const int* __restrict__ grp_off, const int* __restrict__ groups,
+                const int* __restrict__ beta, const int* __restrict__ alpha,
+                const int32_t* __restrict__ active, kparams p)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int cw = blockIdx.x;
+    // frames masked off (active[cw] == 0) keep their outputs; the whole
+    // workgroup leaves before its first barrier
+    if (active && active[cw] == 0) return;
+    const int res = decode_frame<CMAX, EXACT, LANES>(
+        lds_raw, llr_all + (size_t)cw * p.N, msg_all + (size_t)cw * p.q * HALF * rec_t<CMAX>::V4,
+        hard_all + (size_t)cw * (p.K >> 3), circ, lay_cnt, lay_off, steps, grp_off, groups, beta, alpha, p);
+    if (threadIdx.x == 0) result[cw] = res;
 }
 
 }  // namespace ldpc_dev

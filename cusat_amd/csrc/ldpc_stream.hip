@@ -1,0 +1,524 @@
+// Persistent decoder of a mixed ACM stream: frames of several codes (normal
+// and short FECFRAMEs, any MODCOD of the contexts given) in ONE launch.
+//
+// A fixed grid of workgroups (384 threads, one frame at a time each) takes
+// jobs from an atomic queue: a job is a run of frames (same code, consecutive
+// in the plan's order).  Per frame the workgroup
+//   1. decodes with the code's decoder variant (a switch over the compiled
+//      decode_frame instantiations; one-lane short-frame schedules run with
+//      the second lane idle, exactly as the per-code decoder),
+//   2. checks the BCH syndrome of its hard decision with one wave
+//      (bch_check.hip's algorithm, byte table staged in the then free LDS),
+//   3. in learned mode (codes with parameters): a frame that fails
+//      frame_passes_fec_checks (rx_fec_chain.h:193-196) is decoded again
+//      by the plain decoder and checked again, inside the same workgroup.
+// Results equal the per-code launches frame for frame (cusat_ldpc_decode +
+// cusat_bch_check, or cusat_rx_fec_learned in learned mode).
+//
+// Message records live in a per-workgroup workspace (one frame in flight per
+// workgroup), not per frame.  The queue counter resets itself when the last
+// workgroup leaves, so the launch allocates, clears and copies nothing and
+// can be captured in a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <vector>
+
+#include "ctx.h"
+#include "cusat_fec.h"
+#include "kernels.h"
+#include "ldpc_kernel.h"
+
+using namespace cusat;
+
+namespace {
+
+constexpr int STREAM_THREADS = 2 * ldpc_dev::LANE_THREADS;
+
+struct code_desc {
+    const ldpc_dev::circ_rec* circ;
+    const int* lay_cnt;
+    const int* lay_off;
+    const int* steps;
+    const int* grp_off;
+    const int* groups;
+    const int* beta;  // learned parameters, or null
+    const int* alpha;
+    const uint64_t* bch_lut;
+    const uint64_t* bch_pw;
+    int N, K, R, q, n_steps, n_slots, n_groups;
+    int max_trials, learned_trials, n_param_iters;
+    int variant;
+    int bch_bytes, bch_degree, bch_chunk, bch_words;
+};
+
+// two-lane decoder variants in the stream kernel: (id, CMAX, exact)
+#define CUSAT_STREAM_VARIANTS(X)                                                                    \
+    X(0, 2, true) X(1, 3, true) X(2, 4, true) X(3, 5, true) X(4, 7, true) X(5, 8, true)             \
+    X(6, 9, true) X(7, 11, true) X(8, 12, true) X(9, 16, true) X(10, 20, true) X(11, 25, true)     \
+    X(12, 28, true) X(13, 6, false) X(14, 12, false) X(15, 18, false) X(16, 24, false)              \
+    X(17, 30, false)
+
+int variant_id(int cmax, bool exact)
+{
+#define X(ID, C, E) \
+    if (cmax == C && exact == E) return ID;
+    CUSAT_STREAM_VARIANTS(X)
+#undef X
+    return -1;
+}
+
+__device__ __forceinline__ int decode_variant(int v, unsigned char* lds, const int16_t* llr, uint4* rec,
+                                              uint8_t* h, const code_desc& c, const int* beta,
+                                              const int* alpha, const ldpc_dev::kparams& p)
+{
+    switch (v) {
+#define X(ID, C, E)                                                                                   \
+    case ID:                                                                                          \
+        return ldpc_dev::decode_frame<C, E, 2>(lds, llr, rec, h, c.circ, c.lay_cnt, c.lay_off, c.steps, \
+                                               c.grp_off, c.groups, beta, alpha, p);
+        CUSAT_STREAM_VARIANTS(X)
+#undef X
+    }
+    return -1;
+}
+
+// c(x) mod g(x) == 0 over the frame's n_bytes hard bytes, by wave 0 (the
+// algorithm of bch_check.hip: lane chunks folded with the byte table, moved
+// to their place with x^(8 (n - end)) mod g, XOR-reduced over the wave).
+template <int W>
+__device__ __forceinline__ int bch_wave(const uint8_t* __restrict__ src, const uint64_t* tab,
+                                        const uint64_t* __restrict__ pw, int n_bytes, int P, int chunk,
+                                        int lane)
+{
+    constexpr int TOPW = W - 1;  // (P - 8) >> 6 == W - 1 for every DVB-S2 BCH degree
+    const int top_b = (P - 8) & 63;
+    const uint64_t top_mask = (P & 63) ? ((1ull << (P & 63)) - 1) : ~0ull;
+    uint64_t r[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) r[w] = 0;
+    const int begin = lane * chunk;
+    const int end = min(n_bytes, begin + chunk);
+    for (int i = begin; i < end; i++) {
+        const unsigned idx = (unsigned)src[i] ^ (unsigned)((r[TOPW] >> top_b) & 0xffu);
+#pragma unroll
+        for (int w = W - 1; w >= 0; w--) r[w] = (r[w] << 8) | (w ? r[w - 1] >> 56 : 0);
+        r[W - 1] &= top_mask;
+#pragma unroll
+        for (int w = 0; w < W; w++) r[w] ^= tab[idx * W + w];
+    }
+    uint64_t acc[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) acc[w] = 0;
+    if (begin < end) {
+        const uint64_t* pl = pw + (size_t)lane * P * W;
+        for (int b = 0; b < P; b++)
+            if ((r[b >> 6] >> (b & 63)) & 1) {
+#pragma unroll
+                for (int w = 0; w < W; w++) acc[w] ^= pl[b * W + w];
+            }
+    }
+    uint64_t any = 0;
+#pragma unroll
+    for (int w = 0; w < W; w++) {
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) acc[w] ^= __shfl_xor(acc[w], d, 64);
+        any |= acc[w];
+    }
+    return any == 0 ? 1 : 0;
+}
+
+// Whole workgroup: BCH check of the hard bytes at h (written by this
+// workgroup before the call); returns 1 = valid, the same value in every thread.
+__device__ __forceinline__ int bch_frame(unsigned char* lds, int* flag, const uint8_t* __restrict__ h,
+                                         const code_desc& c)
+{
+    uint64_t* tab = reinterpret_cast<uint64_t*>(lds);
+    __syncthreads();  // the decoder is done with the LDS; h is written
+    for (int k = threadIdx.x; k < 256 * c.bch_words; k += STREAM_THREADS) tab[k] = c.bch_lut[k];
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        int ok = 0;
+        switch (c.bch_words) {
+        case 1: ok = bch_wave<1>(h, tab, c.bch_pw, c.bch_bytes, c.bch_degree, c.bch_chunk, threadIdx.x); break;
+        case 2: ok = bch_wave<2>(h, tab, c.bch_pw, c.bch_bytes, c.bch_degree, c.bch_chunk, threadIdx.x); break;
+        case 3: ok = bch_wave<3>(h, tab, c.bch_pw, c.bch_bytes, c.bch_degree, c.bch_chunk, threadIdx.x); break;
+        default: ok = bch_wave<4>(h, tab, c.bch_pw, c.bch_bytes, c.bch_degree, c.bch_chunk, threadIdx.x); break;
+        }
+        if (threadIdx.x == 0) *flag = ok;
+    }
+    __syncthreads();
+    const int ok = *flag;
+    __syncthreads();  // flag read by all before it is reused
+    return ok;
+}
+
+__global__ void __launch_bounds__(STREAM_THREADS)
+    ldpc_stream_kernel(const code_desc* __restrict__ codes, const cusat_stream_frame* __restrict__ frames,
+                       const int2* __restrict__ jobs, const int* __restrict__ order, int n_jobs,
+                       unsigned* counters, const int16_t* __restrict__ llr, uint8_t* __restrict__ hard,
+                       int32_t* __restrict__ trials, int32_t* __restrict__ valid,
+                       int32_t* __restrict__ fallback, uint4* __restrict__ ws, size_t ws_v4, int learned)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ int job_s, flag_s;
+    uint4* rec = ws + (size_t)blockIdx.x * ws_v4;
+    for (;;) {
+        if (threadIdx.x == 0) job_s = (int)atomicAdd(&counters[0], 1u);
+        __syncthreads();
+        const int jb = __builtin_amdgcn_readfirstlane(job_s);
+        __syncthreads();
+        if (jb >= n_jobs) break;
+        const int2 J = jobs[jb];
+        for (int k = 0; k < J.y; k++) {
+            const int f = __builtin_amdgcn_readfirstlane(order[J.x + k]);
+            const int ci = __builtin_amdgcn_readfirstlane(frames[f].code);
+            const code_desc& c = codes[ci];
+            const int16_t* fl = llr + frames[f].llr_offset;
+            uint8_t* fh = hard + frames[f].hard_offset;
+            const bool use_learned = learned && c.learned_trials > 0;
+            int res = -1, ok = 0, fb = 0;
+            for (int pass = 0;; pass++) {
+                const bool lp = use_learned && pass == 0;
+                const ldpc_dev::kparams p{c.N, c.K, c.R, c.q, c.n_steps,
+                                          lp ? c.learned_trials : c.max_trials,
+                                          lp ? c.n_param_iters : 0, c.n_slots, c.n_groups};
+                res = decode_variant(c.variant, lds_raw, fl, rec, fh, c, lp ? c.beta : nullptr,
+                                     lp ? c.alpha : nullptr, p);
+                ok = bch_frame(lds_raw, &flag_s, fh, c);
+                if (!lp || (res >= 0 && ok)) break;
+                fb = 1;
+            }
+            if (threadIdx.x == 0) {
+                trials[f] = res;
+                valid[f] = ok;
+                if (fallback) fallback[f] = fb;
+            }
+        }
+    }
+    // the last workgroup to leave resets the queue for the next launch
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&counters[1], 1u) == gridDim.x - 1) {
+            atomicExch(&counters[0], 0u);
+            atomicExch(&counters[1], 0u);
+        }
+    }
+}
+
+#define HIP_TRY(expr)                                               \
+    do {                                                            \
+        hipError_t e_ = (expr);                                     \
+        if (e_ != hipSuccess) {                                     \
+            set_last_error("%s: %s", #expr, hipGetErrorString(e_)); \
+            return CUSAT_E_HIP;                                     \
+        }                                                           \
+    } while (0)
+
+}  // namespace
+
+struct cusat_stream_ctx {
+    std::vector<cusat_fec_ctx*> codes;
+    std::vector<int> variant;  // stream decoder variant of each code
+    int max_frames = 0;
+    unsigned flags = 0;
+    int n_cu = 0, max_wg_per_cu = 1, lds = 0;
+    size_t ws_v4 = 0;  // uint4 per workgroup
+    // device
+    code_desc* d_codes = nullptr;
+    cusat_stream_frame* d_frames = nullptr;
+    int2* d_jobs = nullptr;
+    int* d_order = nullptr;
+    unsigned* d_counters = nullptr;
+    uint4* d_ws = nullptr;
+    // plan
+    int n_frames = 0, n_jobs = 0;
+    std::vector<cusat_stream_frame> frames;
+    cusat_stream_tuning tuning{};
+};
+
+namespace {
+
+int frame_cost(const cusat_fec_ctx* c)
+{
+    // work of one iteration: steps x (info circulants + parity) of the variant
+    return c->kern_steps * (c->cmax + 2);
+}
+
+int build_plan(cusat_stream_ctx* sc, const cusat_stream_tuning& tu)
+{
+    const int n = sc->n_frames;
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    if (tu.order == CUSAT_STREAM_ORDER_COST)
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            return frame_cost(sc->codes[sc->frames[a].code]) > frame_cost(sc->codes[sc->frames[b].code]);
+        });
+    std::vector<int2> jobs;
+    for (int k = 0; k < n;) {
+        int m = 1;
+        while (m < tu.frames_per_job && k + m < n &&
+               sc->frames[order[k + m]].code == sc->frames[order[k]].code)
+            m++;
+        jobs.push_back(make_int2(k, m));
+        k += m;
+    }
+    sc->n_jobs = (int)jobs.size();
+    if (n) {
+        HIP_TRY(hipMemcpy(sc->d_order, order.data(), n * sizeof(int), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(sc->d_jobs, jobs.data(), jobs.size() * sizeof(int2), hipMemcpyHostToDevice));
+    }
+    sc->tuning = tu;
+    return CUSAT_OK;
+}
+
+int upload_codes(cusat_stream_ctx* sc)
+{
+    std::vector<code_desc> d(sc->codes.size());
+    for (size_t k = 0; k < sc->codes.size(); k++) {
+        const cusat_fec_ctx* c = sc->codes[k];
+        code_desc& o = d[k];
+        memset(&o, 0, sizeof o);
+        o.circ = reinterpret_cast<const ldpc_dev::circ_rec*>(c->d_circ);
+        o.lay_cnt = c->d_lay_cnt;
+        o.lay_off = c->d_lay_off;
+        o.steps = c->d_steps;
+        o.grp_off = c->d_grp_off;
+        o.groups = c->d_groups;
+        o.beta = c->n_param_iters > 0 ? c->d_beta : nullptr;
+        o.alpha = c->n_param_iters > 0 ? c->d_alpha : nullptr;
+        o.bch_lut = c->d_bch_lut;
+        o.bch_pw = c->d_bch_pw;
+        o.N = c->geo.N;
+        o.K = c->geo.K;
+        o.R = c->geo.R;
+        o.q = c->geo.q;
+        o.n_steps = c->kern_steps;
+        o.n_slots = c->geo.n_slots;
+        o.n_groups = (int)c->geo.groups.size();
+        o.max_trials = c->max_trials;
+        o.n_param_iters = c->n_param_iters;
+        o.learned_trials = c->n_param_iters;
+        o.variant = sc->variant[k];
+        o.bch_bytes = c->fp.n_bch / 8;
+        o.bch_degree = c->bch.degree;
+        o.bch_chunk = c->bch_chunk;
+        o.bch_words = (c->bch.degree + 63) / 64;
+    }
+    HIP_TRY(hipMemcpy(sc->d_codes, d.data(), d.size() * sizeof(code_desc), hipMemcpyHostToDevice));
+    return CUSAT_OK;
+}
+
+}  // namespace
+
+extern "C" int cusat_stream_create(cusat_stream_ctx** out, cusat_fec_ctx* const* codes, int n_codes,
+                                   int max_frames, unsigned flags)
+{
+    if (!out || !codes || n_codes < 1 || n_codes > 1024 || max_frames < 1) {
+        set_last_error("bad argument");
+        return CUSAT_E_INVALID;
+    }
+    *out = nullptr;
+    cusat_stream_ctx* sc = new (std::nothrow) cusat_stream_ctx();
+    if (!sc) return CUSAT_E_NOMEM;
+    sc->max_frames = max_frames;
+    sc->flags = flags;
+    size_t rec_max = 0;
+    for (int k = 0; k < n_codes; k++) {
+        cusat_fec_ctx* c = codes[k];
+        if (!c) {
+            delete sc;
+            set_last_error("null context %d", k);
+            return CUSAT_E_INVALID;
+        }
+        int cmax = 0;
+        bool exact = false;
+        int cmin = 1 << 30, cmx = 0;
+        for (int v : c->geo.lay_cnt) {
+            cmin = std::min(cmin, v);
+            cmx = std::max(cmx, v);
+        }
+        // the stream runs every code with a two-lane variant (a one-lane
+        // schedule leaves lane 1 idle), so the variant may differ from the
+        // one the context's own launches use
+        if (!ldpc_select_variant(cmin, cmx, 2, &cmax, &exact) || variant_id(cmax, exact) < 0 ||
+            ((c->bch.degree - 8) >> 6) != (c->bch.degree + 63) / 64 - 1) {
+            delete sc;
+            set_last_error("code %d has no stream decoder variant", k);
+            return CUSAT_E_INVALID;
+        }
+        sc->codes.push_back(c);
+        sc->variant.push_back(variant_id(cmax, exact));
+        rec_max = std::max(rec_max, ldpc_msg_bytes(c->geo.q, cmax));
+        const int lds = ldpc_lds_bytes(c->geo.K, c->geo.R, c->geo.n_slots, c->kern_steps, (int)c->geo.groups.size());
+        sc->lds = std::max(sc->lds, std::max(lds, 256 * 4 * 8));
+    }
+    int dev = 0;
+    hipGetDevice(&dev);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        delete sc;
+        set_last_error("no device properties");
+        return CUSAT_E_HIP;
+    }
+    sc->n_cu = prop.multiProcessorCount;
+    // the attribute is per function: only ever raise it (another context may
+    // need more than this one)
+    static int configured = 0;
+    hipError_t e = hipSuccess;
+    if (sc->lds > __atomic_load_n(&configured, __ATOMIC_ACQUIRE)) {
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&ldpc_stream_kernel),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, sc->lds);
+        if (e == hipSuccess) __atomic_store_n(&configured, sc->lds, __ATOMIC_RELEASE);
+    }
+    int occ = 0;
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, ldpc_stream_kernel, STREAM_THREADS, sc->lds);
+    if (e != hipSuccess || occ < 1) {
+        delete sc;
+        set_last_error("stream kernel does not fit (%d bytes LDS): %s", sc->lds, hipGetErrorString(e));
+        return CUSAT_E_HIP;
+    }
+    sc->max_wg_per_cu = occ;
+    sc->ws_v4 = rec_max / 16;
+    const size_t n_wg = (size_t)sc->n_cu * occ;
+    size_t bytes = sizeof(code_desc) * n_codes + sizeof(cusat_stream_frame) * max_frames +
+                   sizeof(int2) * max_frames + sizeof(int) * max_frames + 256 + 4096;
+    char* base = nullptr;
+    e = hipMalloc(&base, bytes);
+    if (e == hipSuccess) e = hipMalloc(&sc->d_ws, n_wg * rec_max);
+    if (e != hipSuccess) {
+        if (base) hipFree(base);
+        delete sc;
+        set_last_error("stream buffers: %s", hipGetErrorString(e));
+        return CUSAT_E_NOMEM;
+    }
+    auto take = [&](size_t n) {
+        char* p = base;
+        base += (n + 255) & ~(size_t)255;
+        return p;
+    };
+    sc->d_counters = reinterpret_cast<unsigned*>(take(256));
+    sc->d_codes = reinterpret_cast<code_desc*>(take(sizeof(code_desc) * n_codes));
+    sc->d_frames = reinterpret_cast<cusat_stream_frame*>(take(sizeof(cusat_stream_frame) * max_frames));
+    sc->d_jobs = reinterpret_cast<int2*>(take(sizeof(int2) * max_frames));
+    sc->d_order = reinterpret_cast<int*>(take(sizeof(int) * max_frames));
+    if (hipMemset(sc->d_counters, 0, 256) != hipSuccess) {
+        cusat_stream_destroy(sc);
+        set_last_error("counter init failed");
+        return CUSAT_E_HIP;
+    }
+    sc->tuning.wg_per_cu = occ;
+    sc->tuning.frames_per_job = 1;
+    sc->tuning.order = CUSAT_STREAM_ORDER_COST;
+    *out = sc;
+    return CUSAT_OK;
+}
+
+extern "C" int cusat_stream_destroy(cusat_stream_ctx* sc)
+{
+    if (!sc) return CUSAT_OK;
+    if (sc->d_counters) hipFree(sc->d_counters);
+    if (sc->d_ws) hipFree(sc->d_ws);
+    delete sc;
+    return CUSAT_OK;
+}
+
+extern "C" int cusat_stream_plan(cusat_stream_ctx* sc, const cusat_stream_frame* frames, int n_frames,
+                                 const cusat_stream_tuning* tuning)
+{
+    if (!sc || (n_frames > 0 && !frames) || n_frames < 0 || n_frames > sc->max_frames) {
+        set_last_error("bad argument (n_frames %d, max %d)", n_frames, sc ? sc->max_frames : 0);
+        return CUSAT_E_INVALID;
+    }
+    cusat_stream_tuning tu = tuning ? *tuning : sc->tuning;
+    if (tu.wg_per_cu < 1 || tu.wg_per_cu > sc->max_wg_per_cu || tu.frames_per_job < 1 ||
+        tu.frames_per_job > 64 || (tu.order != CUSAT_STREAM_ORDER_GIVEN && tu.order != CUSAT_STREAM_ORDER_COST)) {
+        set_last_error("tuning out of range (wg_per_cu 1..%d, frames_per_job 1..64)", sc->max_wg_per_cu);
+        return CUSAT_E_INVALID;
+    }
+    for (int k = 0; k < n_frames; k++)
+        if (frames[k].code < 0 || frames[k].code >= (int)sc->codes.size() || frames[k].llr_offset < 0 ||
+            frames[k].hard_offset < 0) {
+            set_last_error("frame %d: code %d / offsets out of range", k, frames[k].code);
+            return CUSAT_E_INVALID;
+        }
+    sc->frames.assign(frames, frames + n_frames);
+    sc->n_frames = n_frames;
+    int rc = upload_codes(sc);
+    if (rc != CUSAT_OK) return rc;
+    if (n_frames)
+        HIP_TRY(hipMemcpy(sc->d_frames, frames, n_frames * sizeof(cusat_stream_frame), hipMemcpyHostToDevice));
+    return build_plan(sc, tu);
+}
+
+extern "C" int cusat_stream_get_tuning(const cusat_stream_ctx* sc, cusat_stream_tuning* out, int* max_wg_per_cu)
+{
+    if (!sc || !out) { set_last_error("bad argument"); return CUSAT_E_INVALID; }
+    *out = sc->tuning;
+    if (max_wg_per_cu) *max_wg_per_cu = sc->max_wg_per_cu;
+    return CUSAT_OK;
+}
+
+extern "C" int cusat_stream_decode(cusat_stream_ctx* sc, const int16_t* llr, uint8_t* hard, int32_t* trials,
+                                   int32_t* valid, int32_t* fallback, void* stream)
+{
+    if (!sc || !llr || !hard || !trials || !valid) { set_last_error("bad argument"); return CUSAT_E_INVALID; }
+    if (sc->n_jobs == 0) return CUSAT_OK;
+    const int grid = sc->n_cu * sc->tuning.wg_per_cu;
+    hipLaunchKernelGGL(ldpc_stream_kernel, dim3(grid), dim3(STREAM_THREADS), sc->lds, (hipStream_t)stream,
+                       sc->d_codes, sc->d_frames, sc->d_jobs, sc->d_order, sc->n_jobs, sc->d_counters, llr, hard,
+                       trials, valid, fallback, sc->d_ws, sc->ws_v4,
+                       (sc->flags & CUSAT_STREAM_LEARNED) ? 1 : 0);
+    HIP_TRY(hipGetLastError());
+    return CUSAT_OK;
+}
+
+extern "C" int cusat_stream_autotune(cusat_stream_ctx* sc, const int16_t* llr, uint8_t* hard, int32_t* trials,
+                                     int32_t* valid, int32_t* fallback, int reps, void* stream,
+                                     cusat_stream_tuning* chosen, float* best_ms)
+{
+    if (!sc || reps < 1) { set_last_error("bad argument"); return CUSAT_E_INVALID; }
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    cusat_stream_tuning best = sc->tuning;
+    float best_t = 1e30f;
+    int rc = CUSAT_OK;
+    for (int order = 0; order < 2 && rc == CUSAT_OK; order++)
+        for (int fpj : {1, 2, 4})
+            for (int wg = 1; wg <= sc->max_wg_per_cu && rc == CUSAT_OK; wg++) {
+                cusat_stream_tuning tu{wg, fpj, order, 0};
+                rc = build_plan(sc, tu);
+                if (rc != CUSAT_OK) break;
+                rc = cusat_stream_decode(sc, llr, hard, trials, valid, fallback, stream);  // warm
+                if (rc != CUSAT_OK) break;
+                hipEventRecord(e0, st);
+                for (int r = 0; r < reps && rc == CUSAT_OK; r++)
+                    rc = cusat_stream_decode(sc, llr, hard, trials, valid, fallback, stream);
+                hipEventRecord(e1, st);
+                if (hipEventSynchronize(e1) != hipSuccess) {
+                    set_last_error("autotune: event sync failed");
+                    rc = CUSAT_E_HIP;
+                    break;
+                }
+                float ms = 0;
+                hipEventElapsedTime(&ms, e0, e1);
+                ms /= reps;
+                if (ms < best_t) {
+                    best_t = ms;
+                    best = tu;
+                }
+            }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc != CUSAT_OK) return rc;
+    rc = build_plan(sc, best);
+    if (chosen) *chosen = best;
+    if (best_ms) *best_ms = best_t;
+    return rc;
+}

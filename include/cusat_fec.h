@@ -182,6 +182,57 @@ int cusat_learn_adam(float* param_dev, const float* grad_dev, float* m_dev, floa
                      float lr, float beta1, float beta2, float eps, int step, float lo, float hi,
                      void* stream);
 
+/* ---- mixed ACM stream: one persistent launch for frames of many codes ----
+ * No single reference counterpart: the reference runs one decoder object per
+ * MODCOD (rx_fec_pool.h:26-156 routes frames to per-MODCOD rx_fec_chain
+ * objects, each launching its own demap/decode/BCH per micro-batch).  Here a
+ * stream context holds several MODCOD contexts and decodes an arbitrary mix
+ * of their frames (normal and short FECFRAMEs) in ONE kernel launch, with
+ * the BCH check of every frame fused in; outputs equal cusat_ldpc_decode +
+ * cusat_bch_check of each frame's own context (cusat_rx_fec_learned with
+ * CUSAT_STREAM_LEARNED).  The launch allocates, clears and copies nothing, so
+ * it can be captured in a hipGraph and replayed. */
+typedef struct cusat_stream_ctx cusat_stream_ctx;
+typedef struct cusat_stream_frame {
+    int32_t code;          /* index into the contexts of cusat_stream_create */
+    int32_t reserved;
+    int64_t llr_offset;    /* first LLR of the frame in the llr buffer (elements) */
+    int64_t hard_offset;   /* first hard byte of the frame in the hard buffer (bytes) */
+} cusat_stream_frame;
+#define CUSAT_STREAM_ORDER_GIVEN 0   /* jobs in the frames' order */
+#define CUSAT_STREAM_ORDER_COST  1   /* costliest codes first (shorter tail) */
+typedef struct cusat_stream_tuning {
+    int wg_per_cu;         /* persistent workgroups per CU (1..max the LDS allows) */
+    int frames_per_job;    /* consecutive same-code frames per queue fetch */
+    int order;             /* CUSAT_STREAM_ORDER_* */
+    int reserved;
+} cusat_stream_tuning;
+/* stream flag: codes whose context has learned parameters
+ * (cusat_ldpc_set_params, n_iters > 0) decode n_iters learned iterations,
+ * then every frame failing frame_passes_fec_checks is decoded again by the
+ * plain decoder (the context's max_trials) inside the same launch */
+#define CUSAT_STREAM_LEARNED 0x1u
+/* contexts must outlive the stream context; max_frames bounds a plan */
+int cusat_stream_create(cusat_stream_ctx** out, cusat_fec_ctx* const* codes, int n_codes, int max_frames,
+                        unsigned flags);
+int cusat_stream_destroy(cusat_stream_ctx* sc);
+/* Fix the stream layout (host array, copied) and the tuning (NULL = the
+ * current one; the autotuner's choice after cusat_stream_autotune). */
+int cusat_stream_plan(cusat_stream_ctx* sc, const cusat_stream_frame* frames, int n_frames,
+                      const cusat_stream_tuning* tuning);
+int cusat_stream_get_tuning(const cusat_stream_ctx* sc, cusat_stream_tuning* out, int* max_wg_per_cu);
+/* Decode the planned frames: llr (device int16), hard (device bytes),
+ * trials / valid / fallback (B ints each, fallback may be NULL) indexed by
+ * frame number. */
+int cusat_stream_decode(cusat_stream_ctx* sc, const int16_t* llr, uint8_t* hard, int32_t* trials,
+                        int32_t* valid, int32_t* fallback, void* stream);
+/* Time every tuning candidate (workgroups per CU, frames per job, order)
+ * over reps launches on the given buffers, keep the fastest (results do not
+ * depend on the tuning) and report it with its time per launch. */
+int cusat_stream_autotune(cusat_stream_ctx* sc, const int16_t* llr, uint8_t* hard, int32_t* trials,
+                          int32_t* valid, int32_t* fallback, int reps, void* stream,
+                          cusat_stream_tuning* chosen, float* best_ms);
+
 /* Host-only (no device): the constellation [2 << n_mod floats] and
  * deinterleaver column offsets [8] of a MODCOD, its bits per symbol and
  * interleaver rows (0 for QPSK).  Mirrors make_constellation /
