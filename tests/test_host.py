@@ -211,3 +211,17 @@ def test_frame_acceptance_truth_table():
     t = np.array([25, 0, -1, 3, -1])
     v = np.array([1, 1, 1, 0, 0])
     assert list(ca.frame_passes_fec_checks(t, v)) == [True, True, False, False, False]
+
+
+@pytest.mark.parametrize("workload,n,short,mods", [("mixed12", 2048, 0.30, 12), ("s2x", 4096, 0.25, 15)])
+def test_stream_workload_spec(workload, n, short, mods):
+    from cusat_amd.stream_bench import make_stream
+    spec = make_stream(workload, n, 1)
+    assert len(spec) == n and len({(r, m) for _, r, m, _ in spec}) == mods
+    frac = np.mean([fs == 1 for fs, _, _, _ in spec])
+    assert abs(frac - short) < 0.03
+    # short frames only where a short code exists
+    for fs, r, m, _ in spec:
+        if fs == 1:
+            ca.geometry(1, r, m)
+            assert r not in ("9/10", "135/180")
