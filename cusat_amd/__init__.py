@@ -86,6 +86,10 @@ def lib():
         L.cusat_learn_ctx_destroy.argtypes = [vp]
         L.cusat_learn_fwd_bwd.argtypes = [vp, vp, vp, i, vp, vp, f, u, vp, vp, vp, vp, vp, vp]
         L.cusat_learn_adam.argtypes = [vp, vp, vp, vp, i, f, f, f, f, i, f, f, vp]
+        L.cusat_bb_extract.argtypes = [vp, vp, vp, vp, vp, vp, i, vp]
+        L.cusat_bb_prbs.argtypes = [vp, i]
+        L.cusat_bb_crc8.argtypes = [vp, i]
+        L.cusat_encode_frames.argtypes = [i, i, i, ctypes.c_uint64, i, f, vp, vp, vp]
         L.cusat_stream_create.argtypes = [ctypes.POINTER(vp), vp, i, i, u]
         L.cusat_stream_destroy.argtypes = [vp]
         L.cusat_stream_plan.argtypes = [vp, vp, i, vp]
@@ -185,6 +189,11 @@ class FecContext:
         _check(lib().cusat_ldpc_decode_learned(self._h, _ptr(llr), _ptr(hard), _ptr(trials), batch,
                                                max_trials, _stream(stream)),
                "cusat_ldpc_decode_learned")
+
+    def bb_extract(self, hard, trials, valid, out, dfl, batch, stream=None):
+        """Post-BCH BB descramble + BBHEADER check + GSE data field (GPU)."""
+        _check(lib().cusat_bb_extract(self._h, _ptr(hard), _ptr(trials), _ptr(valid), _ptr(out), _ptr(dfl),
+                                      batch, _stream(stream)), "cusat_bb_extract")
 
     def rx_fec_learned(self, llr, hard, trials, valid, fallback, batch, learned_trials, stream=None):
         _check(lib().cusat_rx_fec_learned(self._h, _ptr(llr), _ptr(hard), _ptr(trials), _ptr(valid),
@@ -335,3 +344,30 @@ def frame_passes_fec_checks(trials, bch_valid):
     (lib/cusat_dsp/rx_fec_chain.h:193-196): the LDPC decoder converged
     (trials >= 0) and the BCH syndrome is zero.  Element-wise on arrays."""
     return (np.asarray(trials) >= 0) & (np.asarray(bch_valid) != 0)
+
+
+def bb_prbs(n_bytes):
+    """The BBFRAME scrambling sequence (EN 302 307-1 clause 5.2.2), bytes MSB first."""
+    out = np.zeros(n_bytes, np.uint8)
+    _check(lib().cusat_bb_prbs(_ptr(out), n_bytes), "cusat_bb_prbs")
+    return out
+
+
+def bb_crc8(data):
+    d = np.ascontiguousarray(data, np.uint8)
+    r = lib().cusat_bb_crc8(_ptr(d), len(d))
+    if r < 0:
+        raise RuntimeError("cusat_bb_crc8 failed")
+    return r
+
+
+def encode_frames(framesize, rate, modulation, seed, info, es_n0_db, want_cw=False):
+    """Transmit chain from given BBFRAMEs uint8 [B, k_bch/8] (host)."""
+    g = geometry(framesize, rate, modulation)
+    info = np.ascontiguousarray(info, np.uint8).reshape(-1, g["k_bch"] // 8)
+    B = info.shape[0]
+    sym = np.zeros((B, g["xfec_len"], 2), np.float32)
+    cw = np.zeros((B, g["n_ldpc"] // 8), np.uint8) if want_cw else None
+    _check(lib().cusat_encode_frames(framesize, rate_id(rate), mod_id(modulation), seed, B, es_n0_db,
+                                     _ptr(info), _ptr(sym), _ptr(cw)), "cusat_encode_frames")
+    return sym, cw

@@ -21,7 +21,7 @@ ARCH = os.environ.get("CUSAT_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["codes.cc", "modulation.cc", "bch.cc", "gen.cc", "capi.cc",
            "ldpc_decode.hip", "ldpc_inst_a.hip", "ldpc_inst_b.hip", "ldpc_inst_c.hip",
-           "ldpc_inst_d.hip", "demap.hip", "bch_check.hip", "learn.hip", "ldpc_stream.hip"]
+           "ldpc_inst_d.hip", "demap.hip", "bch_check.hip", "learn.hip", "ldpc_stream.hip", "bb.hip"]
 
 
 def hipcc():
@@ -65,6 +65,50 @@ def compile_one(src):
     return obj
 
 
+HOST = os.path.join(PKG, "host")
+HOST_SOURCES = ["ldpc_decode.cc", "rx_fec_chain.cc"]
+HOST_LIB = os.path.join(PKG, "libcusat_dsp.so")
+CPP_TESTS = os.path.join(ROOT, "tests", "cpp")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+
+
+def _run(cmd, what):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("%s failed: %s\n%s\n%s" % (what, " ".join(cmd), r.stdout, r.stderr))
+
+
+def build_host(verbose=True):
+    """The C++ cusat_dsp API (ldpc_decode, rx_fec_chain) over the C ABI:
+    libcusat_dsp.so next to libcusat_fec.so, and the C++ test programs of
+    tests/cpp (tests/cpp/_build/<name>)."""
+    flags = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROOT, "include"),
+             "-I", os.path.join(ROCM, "include")]
+    objs = []
+    for src in HOST_SOURCES:
+        obj = os.path.join(BUILD, "host_" + src + ".o")
+        _run(["g++"] + flags + ["-c", os.path.join(HOST, src), "-o", obj], "compile " + src)
+        objs.append(obj)
+    tmp = HOST_LIB + ".tmp"
+    _run(["g++", "-shared", "-o", tmp] + objs + ["-L", PKG, "-l:libcusat_fec.so", "-L", os.path.join(ROCM, "lib"),
+                                                 "-lamdhip64", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(ROCM, "lib")],
+         "link libcusat_dsp.so")
+    os.replace(tmp, HOST_LIB)
+    out = os.path.join(CPP_TESTS, "_build")
+    os.makedirs(out, exist_ok=True)
+    for fn in sorted(os.listdir(CPP_TESTS)):
+        if not fn.endswith(".cc"):
+            continue
+        exe = os.path.join(out, fn[:-3])
+        _run(["g++"] + flags + [os.path.join(CPP_TESTS, fn), "-o", exe + ".tmp", "-L", PKG, "-l:libcusat_dsp.so",
+                                "-l:libcusat_fec.so", "-L", os.path.join(ROCM, "lib"), "-lamdhip64", "-lpthread",
+                                "-Wl,-rpath,$ORIGIN/../../../cusat_amd", "-Wl,-rpath," + os.path.join(ROCM, "lib")],
+             "compile " + fn)
+        os.replace(exe + ".tmp", exe)
+    if verbose:
+        print("built", HOST_LIB)
+
+
 def build(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
     gen_data_header()
@@ -78,6 +122,7 @@ def build(verbose=True):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_host(verbose)
     return LIB
 
 

@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "bb.h"
 #include "bch.h"
 #include "codes.h"
 #include "ctx.h"
@@ -86,6 +87,9 @@ int upload_tables(cusat_fec_ctx* c)
         grp32.insert(grp32.end(), rec, rec + 8);
     }
 
+    std::vector<uint8_t> prbs(c->fp.k_bch / 8);
+    bb_prbs(prbs.data(), prbs.size());
+
     struct piece {
         const void* src;
         size_t bytes;
@@ -102,6 +106,7 @@ int upload_tables(cusat_fec_ctx* c)
         {c->con.xy.data(), (size_t)c->con.n_points * 2 * 4, (void**)&c->d_pts},
         {lut.data(), lut.size() * 8, (void**)&c->d_bch_lut},
         {pw.data(), pw.size() * 8, (void**)&c->d_bch_pw},
+        {prbs.data(), prbs.size(), (void**)&c->d_prbs},
     };
     size_t total = 0;
     for (auto& p : pieces) total += align_up(std::max<size_t>(p.bytes, 16));

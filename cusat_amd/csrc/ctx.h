@@ -35,6 +35,7 @@ struct cusat_fec_ctx {
     int* d_groups = nullptr;
     float* d_pts = nullptr;
     uint64_t *d_bch_lut = nullptr, *d_bch_pw = nullptr;
+    uint8_t* d_prbs = nullptr;  // BB scrambler PRBS, k_bch/8 bytes
     int8_t* d_msg = nullptr;  // max_batch x ldpc_msg_bytes(q, cmax)
     int32_t* d_mask = nullptr;  // [max_batch] fallback mask of cusat_rx_fec_learned
     // learned check-node parameters [n_param_iters][q] (int32 on the device)

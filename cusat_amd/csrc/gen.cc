@@ -44,8 +44,10 @@ void ldpc_encode(const ldpc_table& t, const uint8_t* info, uint8_t* cw)
 
 }  // namespace
 
-extern "C" int cusat_gen_frames(int framesize, int rate, int modulation, uint64_t seed, int batch,
-                                float es_n0_db, float* sym_out, uint8_t* info_out, uint8_t* cw_out)
+namespace {
+
+int gen_impl(int framesize, int rate, int modulation, uint64_t seed, int batch, float es_n0_db,
+             const uint8_t* info_in, float* sym_out, uint8_t* info_out, uint8_t* cw_out)
 {
     fec_params fp;
     const char* tname = ldpc_table_name(framesize, rate);
@@ -81,7 +83,10 @@ extern "C" int cusat_gen_frames(int framesize, int rate, int modulation, uint64_
         for (int b = b0; b < b1; b++) {
             std::seed_seq ss{(uint32_t)seed, (uint32_t)(seed >> 32), (uint32_t)b};
             std::mt19937 rng(ss);
-            for (auto& x : info) x = (uint8_t)(rng() & 0xff);
+            if (info_in)
+                memcpy(info.data(), info_in + (size_t)b * info.size(), info.size());
+            else
+                for (auto& x : info) x = (uint8_t)(rng() & 0xff);
             bch_encode(bch, info.data(), bcw.data());
             std::fill(cw.begin(), cw.end(), 0);
             ldpc_encode(*tab, bcw.data(), cw.data());
@@ -121,4 +126,22 @@ extern "C" int cusat_gen_frames(int framesize, int rate, int modulation, uint64_
     }
     for (auto& x : th) x.join();
     return CUSAT_OK;
+}
+
+}  // namespace
+
+extern "C" int cusat_gen_frames(int framesize, int rate, int modulation, uint64_t seed, int batch,
+                                float es_n0_db, float* sym_out, uint8_t* info_out, uint8_t* cw_out)
+{
+    return gen_impl(framesize, rate, modulation, seed, batch, es_n0_db, nullptr, sym_out, info_out, cw_out);
+}
+
+extern "C" int cusat_encode_frames(int framesize, int rate, int modulation, uint64_t seed, int batch,
+                                   float es_n0_db, const uint8_t* info_in, float* sym_out, uint8_t* cw_out)
+{
+    if (!info_in) {
+        set_last_error("cusat_encode_frames: null input");
+        return CUSAT_E_INVALID;
+    }
+    return gen_impl(framesize, rate, modulation, seed, batch, es_n0_db, info_in, sym_out, nullptr, cw_out);
 }

@@ -182,6 +182,23 @@ int cusat_learn_adam(float* param_dev, const float* grad_dev, float* m_dev, floa
                      float lr, float beta1, float beta2, float eps, int step, float lo, float hi,
                      void* stream);
 
+/* ---- post-BCH baseband step ----
+ * Replaces the per-frame host loop of rx_fec_chain::post_process_slot
+ * (rx_fec_chain.cc:565-627): for every frame that passes
+ * frame_passes_fec_checks, BB descrambling (bb_scrambler.cc), the BBHEADER
+ * CRC-8 and field checks and the GSE data-field extraction
+ * (bbframe_parser.cc:13-56), on the GPU.  hard / trials / valid as the
+ * decoder and the BCH check wrote them (B frames); out = B x (k_bch/8 - 10)
+ * bytes, frame b's data field at b (k_bch/8 - 10); dfl_out = B ints:
+ * -1 = frame failed the FEC checks, 0 = no data field (header CRC or field
+ * error, TS/GS not generic packetized, or DFL 0), n > 0 = data-field bytes. */
+int cusat_bb_extract(cusat_fec_ctx* ctx, const uint8_t* hard, const int32_t* trials, const int32_t* valid,
+                     uint8_t* out, int32_t* dfl_out, int batch, void* stream);
+/* Host: the BBFRAME scrambling sequence (first n_bytes, MSB first) and the
+ * BBHEADER CRC-8 of n_bytes (returns the CRC, 0..255). */
+int cusat_bb_prbs(uint8_t* out, int n_bytes);
+int cusat_bb_crc8(const uint8_t* data, int n_bytes);
+
 /* ---- mixed ACM stream: one persistent launch for frames of many codes ----
  * No single reference counterpart: the reference runs one decoder object per
  * MODCOD (rx_fec_pool.h:26-156 routes frames to per-MODCOD rx_fec_chain
@@ -255,6 +272,12 @@ int cusat_bch_encode(int framesize, int rate, const uint8_t* msg, uint8_t* cw, i
  * es_n0_db >= 1000. */
 int cusat_gen_frames(int framesize, int rate, int modulation, uint64_t seed, int batch,
                      float es_n0_db, float* sym_out, uint8_t* info_out, uint8_t* cw_out);
+
+/* The transmit chain of cusat_gen_frames from caller-given BBFRAMEs
+ * (info_in = B x k_bch/8 bytes) instead of random bytes; the noise of frame
+ * b is still seeded by (seed, b). */
+int cusat_encode_frames(int framesize, int rate, int modulation, uint64_t seed, int batch,
+                        float es_n0_db, const uint8_t* info_in, float* sym_out, uint8_t* cw_out);
 
 #ifdef __cplusplus
 }

@@ -225,3 +225,42 @@ def test_stream_workload_spec(workload, n, short, mods):
         if fs == 1:
             ca.geometry(1, r, m)
             assert r not in ("9/10", "135/180")
+
+
+def test_bb_prbs_matches_register_definition():
+    """qa_bb_scrambler.cc:38-57: first output bits 0 0 0 0 0 0 1 1; every
+    byte equals the bit-serial register of figure 5, MSB first."""
+    from tests.bb_ref import prbs_bits
+    n = 7274
+    seq = ca.bb_prbs(n)
+    assert seq[0] == 0x03
+    bits = prbs_bits(8 * n)
+    assert list(bits[:8]) == [0, 0, 0, 0, 0, 0, 1, 1]
+    np.testing.assert_array_equal(seq, np.packbits(bits))
+
+
+def test_bb_crc8_kats():
+    """qa_crc8.cc:12-29: CRC-8/DVB-S2 check value, appended CRC checks to 0,
+    single-bit errors detected."""
+    from tests.bb_ref import crc8
+    assert ca.bb_crc8(np.frombuffer(b"123456789", np.uint8)) == 0xBC
+    hdr = np.array([0xF2, 0x00, 0x00, 0x00, 0x7E, 0x10, 0x00, 0x00, 0x00], np.uint8)
+    full = np.append(hdr, ca.bb_crc8(hdr)).astype(np.uint8)
+    assert ca.bb_crc8(full) == 0
+    for bit in range(80):
+        f = full.copy()
+        f[bit // 8] ^= 0x80 >> (bit % 8)
+        assert ca.bb_crc8(f) != 0
+    rng = np.random.default_rng(4)
+    for _ in range(50):
+        d = rng.integers(0, 256, int(rng.integers(0, 40)), dtype=np.uint8)
+        assert ca.bb_crc8(d) == crc8(d)
+
+
+def test_encode_frames_is_gen_frames_chain():
+    """cusat_encode_frames on the info bytes gen_frames drew gives the same
+    codewords (same BCH / LDPC / map), noiseless."""
+    s1, info, cw1 = ca.gen_frames(0, "2/3", "16APSK", 9, 3, 1000.0, want_cw=True)
+    s2, cw2 = ca.encode_frames(0, "2/3", "16APSK", 9, info, 1000.0, want_cw=True)
+    np.testing.assert_array_equal(cw1, cw2)
+    np.testing.assert_array_equal(s1, s2)
