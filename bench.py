@@ -41,6 +41,7 @@ def parse():
     ap.add_argument("--cpu-frames", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", type=int, default=8, help="frames checked against the oracle")
+    ap.add_argument("--no-split", action="store_true", help="decode with the 6-wave kernel")
     return ap.parse_args()
 
 
@@ -55,7 +56,8 @@ def main():
 
     fs, rate, mod = 0, "3/4", "8PSK"
     B, T, es = args.batch, args.trials, args.esn0
-    ctx = ca.FecContext(fs, rate, mod, max_trials=T, max_batch=B)
+    ctx = ca.FecContext(fs, rate, mod, max_trials=T, max_batch=B,
+                        flags=ca.CTX_NO_SPLIT_KERNEL if args.no_split else 0)
     inf = ctx.info
     N, K, kb = inf["n_ldpc"], inf["k_ldpc"], inf["k_bch"]
     sym, info_bits, _ = ca.gen_frames(fs, rate, mod, 20240607, B, es)

@@ -24,6 +24,7 @@ RATES = ["1/4", "1/3", "2/5", "1/2", "3/5", "2/3", "3/4", "4/5", "5/6", "8/9", "
          "11/20", "100/180", "26/45", "104/180", "18/30", "28/45", "23/36", "116/180",
          "20/30", "124/180", "25/36", "32/45", "128/180", "13/18", "22/30", "132/180",
          "135/180", "7/9", "140/180", "154/180"]
+CTX_NO_SPLIT_KERNEL = 0x2
 CTX_SEQUENTIAL_SCHEDULE = 0x1
 
 

@@ -1,6 +1,7 @@
 // C ABI (include/cusat_fec.h): contexts, device tables, launches.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -150,6 +151,7 @@ ldpc_args make_ldpc_args(const cusat_fec_ctx* c, const int16_t* llr, uint8_t* ha
     a.max_trials = c->max_trials;
     a.cmax = c->cmax;
     a.exact = c->exact;
+    a.split = !(c->flags & CUSAT_CTX_NO_SPLIT_KERNEL);
     return a;
 }
 
@@ -199,6 +201,7 @@ namespace {
 int prepare(cusat_fec_ctx* c, int framesize, int rate, int modulation, unsigned flags)
 {
     c->framesize = framesize;
+    c->flags = flags;
     c->rate = rate;
     c->modulation = modulation;
     const char* tname = ldpc_table_name(framesize, rate);
@@ -254,8 +257,10 @@ void fill_info(const cusat_fec_ctx* c, cusat_fec_info* o)
     o->max_batch = c->max_batch;
     o->max_trials = c->max_trials;
     o->n_collision_slots = c->geo.n_slots;
-    o->lds_bytes = ldpc_lds_bytes(c->geo.K, c->geo.R, c->geo.n_slots, c->geo.n_steps + (c->geo.n_steps & 1),
-                                  (int)c->geo.groups.size());
+    const int ks = c->geo.n_steps + (c->geo.n_steps & 1), ng = (int)c->geo.groups.size();
+    o->lds_bytes = (!(c->flags & CUSAT_CTX_NO_SPLIT_KERNEL) && ldpc_split_available(c->cmax, c->exact, c->geo.lanes))
+                       ? ldpc_lds_bytes_split(c->geo.K, c->geo.R, c->geo.n_slots, ks, ng, c->geo.lanes)
+                       : ldpc_lds_bytes(c->geo.K, c->geo.R, c->geo.n_slots, ks, ng);
 }
 
 }  // namespace
@@ -295,7 +300,8 @@ extern "C" int cusat_fec_ctx_create(cusat_fec_ctx** out, int framesize, int rate
     hipGetDevice(&c->device);
     rc = upload_tables(c);
     if (rc == CUSAT_OK) {
-        hipError_t e = hipMalloc(&c->d_msg, (size_t)max_batch * ldpc_msg_bytes(c->geo.q, c->cmax));
+        const size_t rec = std::max(ldpc_msg_bytes(c->geo.q, c->cmax), ldpc_msg_bytes_split(c->geo.q, c->cmax));
+        hipError_t e = hipMalloc(&c->d_msg, (size_t)max_batch * rec);
         if (e == hipSuccess) e = hipMalloc(&c->d_mask, (size_t)max_batch * sizeof(int32_t));
         if (e != hipSuccess) {
             set_last_error("message workspace: %s", hipGetErrorString(e));

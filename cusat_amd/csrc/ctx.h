@@ -15,6 +15,7 @@ void set_last_error(const char* fmt, ...);
 
 struct cusat_fec_ctx {
     int framesize, rate, modulation, max_trials, max_batch;
+    unsigned flags = 0;
     cusat::fec_params fp{};
     cusat::decoder_geometry geo;
     int cmax = 0;      // kernel variant: info circulants per layer

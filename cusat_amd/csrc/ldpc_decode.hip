@@ -44,8 +44,38 @@ int ldpc_lds_bytes(int K, int R, int n_slots, int n_steps, int n_groups)
     return ((K + R) * 2 + n_slots * CIRC * 2 + 4 * (3 * n_steps + 1 + 8 * n_groups) + 15) & ~15;
 }
 
+bool ldpc_split_available(int cmax, bool exact, int lanes)
+{
+    if (!exact) return false;
+    const int* b = lanes == 2 ? EXACT2 : EXACT1;
+    const int n = lanes == 2 ? (int)(sizeof(EXACT2) / sizeof(int)) : (int)(sizeof(EXACT1) / sizeof(int));
+    for (int k = 0; k < n; k++)
+        if (b[k] == cmax) return true;
+    return false;
+}
+
+int ldpc_lds_bytes_split(int K, int R, int n_slots, int n_steps, int n_groups, int lanes)
+{
+    return ldpc_lds_bytes(K, R, n_slots, n_steps, n_groups) + 3 * lanes * 2 * 180 * 4;
+}
+
+size_t ldpc_msg_bytes_split(int q, int cmax)
+{
+    const int neh = (cmax + 1) / 2 + 1;
+    const int words = (((neh + 1) / 2) + 3) / 4 * 4;
+    return (size_t)q * 2 * 180 * words * 4;
+}
+
 hipError_t launch_ldpc_decode(const ldpc_args& a, int batch, hipStream_t stream)
 {
+    if (a.split && ldpc_split_available(a.cmax, a.exact, a.lanes)) {
+#define SCASE(C, L) \
+    if (a.cmax == C && a.lanes == L) return launch_ldpc_split_variant<C, L>(a, batch, stream);
+        SCASE(2, 2) SCASE(3, 2) SCASE(4, 2) SCASE(5, 2) SCASE(7, 2) SCASE(8, 2) SCASE(9, 2) SCASE(11, 2)
+        SCASE(12, 2) SCASE(16, 2) SCASE(20, 2) SCASE(25, 2) SCASE(28, 2)
+        SCASE(2, 1) SCASE(3, 1) SCASE(4, 1) SCASE(8, 1) SCASE(9, 1) SCASE(11, 1) SCASE(25, 1)
+#undef SCASE
+    }
 #define CASE(C, E, L) \
     if (a.cmax == C && a.exact == E && a.lanes == L) return launch_ldpc_variant<C, E, L>(a, batch, stream);
     CASE(2, true, 2) CASE(3, true, 2) CASE(4, true, 2) CASE(5, true, 2) CASE(7, true, 2)

@@ -1,0 +1,369 @@
+// Split-edge variant of the layered min-sum decode kernel (device code;
+// instantiated in ldpc_inst_s.hip).  Same arithmetic, LDS layout and
+// collision handling as ldpc_kernel.h, with the work of a check pair divided
+// between two groups of waves so that a workgroup runs 12 waves (3 per
+// SIMD) instead of 6 (2, 2, 1, 1):
+//
+//  * A lane (layer of the schedule step) is 6 waves: waves 0-2 ("half 0")
+//    evaluate data edges 0 .. NE0-1 and the parity edge P(i, j) of the check
+//    pairs (t, t + 180), waves 3-5 ("half 1") data edges NE0 .. CMAX-1 and
+//    the second parity edge.  Every wave still walks one circulant at a time,
+//    so circulant records stay scalar loads.
+//  * Each half reduces its edges to (min1, min2, sign) of |x| -- the offset
+//    beta is applied once per check, max(min|x| - beta, 0) being the minimum
+//    of max(|x| - beta, 0) -- and the halves swap these partials through LDS
+//    (one barrier); then each half writes its own edges.
+//  * A message record holds one half's edges: [layer][half][pair] x 16 B
+//    multiples, so a wave still moves consecutive 16-byte records.
+//
+// Reads of a step see the step-start APP and every variable node is written
+// once per step outside the collision slots, as in ldpc_kernel.h, so the
+// results are the same bit for bit.
+#pragma once
+
+#include "ldpc_kernel.h"
+
+namespace cusat {
+namespace ldpc_dev {
+
+constexpr int SPLIT_LANE_THREADS = 2 * LANE_THREADS;  // 6 waves per lane
+
+// message record of one half: (CMAX + 1) / 2 data slots + 1 parity slot
+template <int CMAX>
+struct rec_half_t {
+    static constexpr int NE0 = (CMAX + 1) / 2;
+    static constexpr int NE1 = CMAX - NE0;
+    static constexpr int NEH = NE0 + 1;  // parity edge in the last slot
+    static constexpr int WORDS = (((NEH + 1) / 2) + 3) / 4 * 4;
+    static constexpr int V4 = WORDS / 4;
+};
+
+template <int CMAX, int LANES>
+__device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const int16_t* __restrict__ llr,
+                                                  uint4* __restrict__ rec_base, uint8_t* __restrict__ h,
+                                                  const circ_rec* __restrict__ circ,
+                                                  const int* __restrict__ lay_off,
+                                                  const int* __restrict__ steps, const int* __restrict__ grp_off,
+                                                  const int* __restrict__ groups, const int* __restrict__ beta,
+                                                  const int* __restrict__ alpha, const kparams& p)
+{
+    using R = rec_half_t<CMAX>;
+    constexpr int NE0 = R::NE0, NE1 = R::NE1, NEH = R::NEH;
+    uint32_t* app32 = reinterpret_cast<uint32_t*>(lds_raw);  // K/2 dwords
+    uint32_t* par32 = app32 + p.K / 2;                        // R/2 dwords
+    uint32_t* dsl32 = par32 + p.R / 2;                        // n_slots * 180 dwords
+    uint32_t* xch = dsl32 + p.n_slots * HALF;                 // [3][LANES][2][180] partials
+    int* sched_l = reinterpret_cast<int*>(xch + 3 * LANES * 2 * HALF);
+    int* goff_l = sched_l + 2 * p.n_steps;
+    int* grp_l = goff_l + p.n_steps + 1;
+    int16_t* app16 = reinterpret_cast<int16_t*>(lds_raw);
+    int16_t* par16 = reinterpret_cast<int16_t*>(par32);
+
+    constexpr int NTHR = LANES * SPLIT_LANE_THREADS;
+    const int t = threadIdx.x;
+    const int lane = LANES == 2 ? __builtin_amdgcn_readfirstlane(t >= SPLIT_LANE_THREADS ? 1 : 0) : 0;
+    const int u = t - lane * SPLIT_LANE_THREADS;
+    const int half = __builtin_amdgcn_readfirstlane(u >= LANE_THREADS ? 1 : 0);
+    const int tt = u - half * LANE_THREADS;
+    const int tt4 = 4 * tt;
+    const bool act = tt < HALF;
+    const int q = p.q;
+    const int quarter = lane * 2 + half;  // which (lane, half) this wave group is
+    uint32_t* xm1 = xch;
+    uint32_t* xm2 = xch + LANES * 2 * HALF;
+    uint32_t* xsg = xch + 2 * LANES * 2 * HALF;
+
+    for (int k = t; k < 2 * p.n_steps; k += NTHR) sched_l[k] = steps[k];
+    for (int k = t; k <= p.n_steps; k += NTHR) goff_l[k] = grp_off[k];
+    for (int k = t; k < GROUP_WORDS * p.n_groups; k += NTHR) grp_l[k] = groups[k];
+    // input clamp of the int16 APP (ldpc_decode_kernel.cu:655-670): -32768 -> -32767
+    for (int v = t; v < p.K; v += NTHR) app16[vn_index(v)] = (int16_t)max((int)llr[v], -32767);
+    for (int c = t; c < p.R; c += NTHR) {
+        const int i = c % q, j = c / q;
+        par16[i * CIRC + (j < HALF ? 2 * j : 2 * (j - HALF) + 1)] = (int16_t)max((int)llr[p.K + c], -32767);
+    }
+    __syncthreads();
+
+    const int tl = act ? tt : HALF - 1;
+    auto rec_at = [&](int i, int pair) { return rec_base + (((size_t)i * 2 + half) * HALF + pair) * R::V4; };
+    for (int s = 0; s < p.n_steps; s++) {
+        const int i = __builtin_amdgcn_readfirstlane(sched_l[2 * s + lane]);
+        if (act && i >= 0) {
+            uint4* dst = rec_at(i, tt);
+#pragma unroll
+            for (int k = 0; k < R::V4; k++) dst[k] = make_uint4(0, 0, 0, 0);
+        }
+    }
+    uint4 ra[R::V4], rb[R::V4];
+    {
+        const int i0 = max(__builtin_amdgcn_readfirstlane(sched_l[lane]), 0);
+        const uint4* src = rec_at(i0, tl);
+#pragma unroll
+        for (int k = 0; k < R::V4; k++) {
+            ra[k] = src[k];
+            rb[k] = make_uint4(0, 0, 0, 0);
+        }
+    }
+
+    int res = -1;
+    for (int it = 0;; it++) {
+        // ---- syndrome (half 0 evaluates whole check pairs), early exit ----
+        int ok = 1;
+        for (int s = 0; s < p.n_steps; s++) {
+            const int i = __builtin_amdgcn_readfirstlane(sched_l[2 * s + lane]);
+            int bad = 0;
+            if (act && i >= 0 && half == 0) {
+                uint32_t sg = par32[i * HALF + tt];
+                if (i > 0) {
+                    sg ^= par32[(i - 1) * HALF + tt];
+                } else {
+                    const uint32_t w = par32[(q - 1) * HALF + (tt > 0 ? tt - 1 : HALF - 1)];
+                    sg ^= tt > 0 ? w : (w << 16);
+                }
+                const circ_rec* cc = circ + lay_off[i];
+#pragma unroll
+                for (int e = 0; e < CMAX; e++) {
+                    const circ_rec c = cc[e];
+                    int off;
+                    uint32_t sel;
+                    pair_addr(c, tt4, tt, off, sel);
+                    sg ^= perm(lds_ld(lds_raw, c.app_off + off), sel);
+                }
+                bad = (sg & 0x80008000u) != 0;
+            }
+            if (__syncthreads_or(bad)) { ok = 0; break; }
+        }
+        if (ok) { res = p.max_trials - it; break; }
+        if (it == p.max_trials) break;
+
+        // ---- one layered iteration ----
+        for (int s0 = 0; s0 < p.n_steps; s0 += 2) {
+#pragma unroll
+        for (int hs = 0; hs < 2; hs++) {
+            const int s = s0 + hs;
+            uint4* cur = hs == 0 ? ra : rb;
+            uint4* nxt = hs == 0 ? rb : ra;
+            const int i = __builtin_amdgcn_readfirstlane(sched_l[2 * s + lane]);
+            uint32_t rw[R::WORDS];
+#pragma unroll
+            for (int k = 0; k < R::V4; k++) {
+                rw[4 * k] = cur[k].x;
+                rw[4 * k + 1] = cur[k].y;
+                rw[4 * k + 2] = cur[k].z;
+                rw[4 * k + 3] = cur[k].w;
+            }
+            {
+                const int sn = s + 1 < p.n_steps ? s + 1 : 0;
+                const int inx = max(__builtin_amdgcn_readfirstlane(sched_l[2 * sn + lane]), 0);
+                const uint4* src = rec_at(inx, tl);
+#pragma unroll
+                for (int k = 0; k < R::V4; k++) nxt[k] = src[k];
+            }
+            const bool work = act && i >= 0;
+            const s2 zero = splat(0);
+            const int ne = half ? NE1 : NE0;  // data edges of this half (wave-uniform)
+            s2 xv[NE0], mv[NE0], av[NE0], ov[NE0];
+            int wa[NE0];
+            uint32_t sels[NE0];
+            uint32_t slotm = 0;
+            s2 mn1 = splat(32767), mn2 = splat(32767);
+            uint32_t sg = 0;
+            // parity edge: half 0 the own P(i, j), half 1 P(i-1, j) / P(q-1, j-1);
+            // check (0, 0) has no second parity edge: its A half reads a +32767
+            // placeholder that cannot change min1/min2 or the sign
+            const int pidx = half == 0 ? i * HALF + tt
+                                       : (i > 0 ? (i - 1) * HALF + tt : (q - 1) * HALF + (tt > 0 ? tt - 1 : HALF - 1));
+            const bool dummy = half == 1 && i == 0 && tt == 0;
+            s2 xp = zero, mp = zero;
+            if (work) {
+                const circ_rec* cc = circ + lay_off[i] + half * NE0;
+#pragma unroll
+                for (int e = 0; e < NE0; e++) {
+                    if (e < ne) {
+                        const circ_rec c = cc[e];
+                        int off;
+                        pair_addr(c, tt4, tt, off, sels[e]);
+                        const s2 ap = as_s2(perm(lds_ld(lds_raw, c.app_off + off), sels[e]));
+                        wa[e] = (c.slot_off < 0 ? c.app_off : c.slot_off) + off;
+                        slotm |= (c.slot_off < 0 ? 0u : 1u) << e;
+                        const s2 o = unpack_msg(rw, e);
+                        const s2 x = __builtin_elementwise_sub_sat(ap, o);
+                        const s2 m = __builtin_elementwise_max(x, __builtin_elementwise_sub_sat(zero, x));
+                        mn2 = __builtin_elementwise_min(mn2, __builtin_elementwise_max(mn1, m));
+                        mn1 = __builtin_elementwise_min(mn1, m);
+                        sg ^= bits(x);
+                        av[e] = ap;
+                        ov[e] = o;
+                        xv[e] = x;
+                        mv[e] = m;
+                    }
+                }
+                {
+                    const s2 o = unpack_msg(rw, NEH - 1);
+                    uint32_t w = par32[pidx];
+                    if (dummy) w = (w << 16) | 0x7fffu;
+                    xp = __builtin_elementwise_sub_sat(as_s2(w), o);
+                    mp = __builtin_elementwise_max(xp, __builtin_elementwise_sub_sat(zero, xp));
+                    mn2 = __builtin_elementwise_min(mn2, __builtin_elementwise_max(mn1, mp));
+                    mn1 = __builtin_elementwise_min(mn1, mp);
+                    sg ^= bits(xp);
+                }
+                const int xo = quarter * HALF + tt;
+                xm1[xo] = bits(mn1);
+                xm2[xo] = bits(mn2);
+                xsg[xo] = sg;
+            }
+            __syncthreads();  // every read of the step is done; partials are in LDS
+            if (work) {
+                {
+                    const int po = (quarter ^ 1) * HALF + tt;  // the other half of the same lane
+                    const s2 om1 = as_s2(xm1[po]), om2 = as_s2(xm2[po]);
+                    mn2 = __builtin_elementwise_min(__builtin_elementwise_min(mn2, om2),
+                                                    __builtin_elementwise_max(mn1, om1));
+                    mn1 = __builtin_elementwise_min(mn1, om1);
+                    sg ^= xsg[po];
+                }
+                int bt = 1, al = 0;
+                if (beta && it < p.n_param_iters) bt = beta[it * q + i];
+                if (alpha && it < p.n_param_iters) al = alpha[it * q + i];
+                const u2 beta2 = as_u2(splat(bt));
+                // offset once per check: max(min|x| - beta, 0)
+                const s2 b1 = as_s2u(__builtin_elementwise_sub_sat(as_u2(mn1), beta2));
+                const s2 b2 = as_s2u(__builtin_elementwise_sub_sat(as_u2(mn2), beta2));
+                u2 c1, c2;
+                if (al == 0) {
+                    c1 = __builtin_elementwise_min(as_u2(b1), (u2)127);
+                    c2 = __builtin_elementwise_min(as_u2(b2), (u2)127);
+                } else {
+                    s2 a1, a2;
+                    a1.x = (short)min(nms_mag(b1.x, al), 127);
+                    a1.y = (short)min(nms_mag(b1.y, al), 127);
+                    a2.x = (short)min(nms_mag(b2.x, al), 127);
+                    a2.y = (short)min(nms_mag(b2.y, al), 127);
+                    c1 = as_u2(a1);
+                    c2 = as_u2(a2);
+                }
+                __asm__("" : "+v"(sg));
+                // |x_e| == min1 exactly on the arg-min edge (all |x| >= min1)
+                const u2 mn1p1 = as_u2(mn1) + (u2)1;
+                const uint32_t c1b = bits(as_s2u(c1)), c2b = bits(as_s2u(c2));
+                auto out = [&](s2 m, s2 x) {
+                    const u2 hit = __builtin_elementwise_sub_sat(mn1p1, as_u2(m));
+                    const uint32_t mask = bits(as_s2u((u2)0 - hit));
+                    const s2 mag = as_s2((mask & c2b) | (~mask & c1b));
+                    const s2 sgn = as_s2(sg ^ bits(x)) >> (short)15;
+                    return (mag ^ sgn) - sgn;
+                };
+                s2 ys[NEH];
+#pragma unroll
+                for (int e = 0; e < NE0; e++) {
+                    ys[e] = zero;
+                    if (e < ne) {
+                        const s2 y = out(mv[e], xv[e]);
+                        ys[e] = y;
+                        const s2 d = y - ov[e];
+                        if ((slotm >> e) & 1u)
+                            lds_st(lds_raw, wa[e], perm(bits(d), sels[e]));
+                        else
+                            lds_st(lds_raw, wa[e], perm(bits(__builtin_elementwise_add_sat(av[e], d)), sels[e]));
+                    }
+                }
+                {
+                    s2 yp = out(mp, xp);
+                    const s2 np = __builtin_elementwise_add_sat(xp, yp);
+                    if (!dummy) {
+                        par32[pidx] = bits(np);
+                    } else {
+                        par16[2 * pidx] = np.y;  // low half of dword P(q-1, 179)
+                        yp.x = 0;
+                    }
+                    ys[NEH - 1] = yp;
+                }
+                uint32_t ow[R::WORDS];
+#pragma unroll
+                for (int k = 0; k < R::WORDS; k++)
+                    ow[k] = 2 * k + 1 < NEH ? pack_msg(ys[2 * k], ys[2 * k + 1])
+                          : 2 * k < NEH     ? pack_msg(ys[2 * k], zero)
+                                            : 0u;
+                uint4* dst = rec_at(i, tt);
+#pragma unroll
+                for (int k = 0; k < R::V4; k++)
+                    dst[k] = make_uint4(ow[4 * k], ow[4 * k + 1], ow[4 * k + 2], ow[4 * k + 3]);
+            }
+            __syncthreads();
+            const int g0 = __builtin_amdgcn_readfirstlane(goff_l[s]);
+            const int g1 = __builtin_amdgcn_readfirstlane(goff_l[s + 1]);
+            if (g1 > g0) {
+                for (int gb = g0; gb < g1; gb += 2 * LANES) {
+                    const int g = gb + quarter;
+                    if (g < g1 && act) {
+                        const int* gr = grp_l + GROUP_WORDS * g;
+                        const int base = __builtin_amdgcn_readfirstlane(gr[0]);
+                        const int n = __builtin_amdgcn_readfirstlane(gr[1]);
+                        s2 val = as_s2(app32[base + tt]);
+                        for (int e = 0; e < n; e++)
+                            val = __builtin_elementwise_add_sat(
+                                val, as_s2(app32[__builtin_amdgcn_readfirstlane(gr[2 + e]) + tt]));
+                        app32[base + tt] = bits(val);
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        }
+    }
+
+    for (int b = t; b < (p.K >> 3); b += NTHR) {
+        int byte = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) byte |= (app16[vn_index(8 * b + k)] < 0 ? 1 : 0) << (7 - k);
+        h[b] = (uint8_t)byte;
+    }
+    return res;
+}
+
+template <int CMAX, int LANES>
+__global__ void __launch_bounds__(LANES* SPLIT_LANE_THREADS)
+    ldpc_kernel_split(const int16_t* __restrict__ llr_all, uint4* __restrict__ msg_all,
+                      uint8_t* __restrict__ hard_all, int32_t* __restrict__ result,
+                      const circ_rec* __restrict__ circ, const int* __restrict__ lay_off,
+                      const int* __restrict__ steps, const int* __restrict__ grp_off,
+                      const int* __restrict__ groups, const int* __restrict__ beta,
+                      const int* __restrict__ alpha, const int32_t* __restrict__ active, kparams p)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int cw = blockIdx.x;
+    if (active && active[cw] == 0) return;
+    const int res = decode_frame_split<CMAX, LANES>(
+        lds_raw, llr_all + (size_t)cw * p.N, msg_all + (size_t)cw * p.q * 2 * HALF * rec_half_t<CMAX>::V4,
+        hard_all + (size_t)cw * (p.K >> 3), circ, lay_off, steps, grp_off, groups, beta, alpha, p);
+    if (threadIdx.x == 0) result[cw] = res;
+}
+
+}  // namespace ldpc_dev
+
+template <int CMAX, int LANES>
+hipError_t launch_ldpc_split_variant(const ldpc_args& a, int batch, hipStream_t stream)
+{
+    auto kern = &ldpc_dev::ldpc_kernel_split<CMAX, LANES>;
+    const int lds = ldpc_lds_bytes_split(a.K, a.R, a.n_slots, a.n_steps, a.n_groups, LANES);
+    static int configured = 0;
+    if (lds > __atomic_load_n(&configured, __ATOMIC_ACQUIRE)) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        if (e != hipSuccess) return e;
+        __atomic_store_n(&configured, lds, __ATOMIC_RELEASE);
+    }
+    const ldpc_dev::kparams p{a.N, a.K, a.R, a.q, a.n_steps, a.max_trials, a.n_param_iters,
+                              a.n_slots, a.n_groups};
+    hipLaunchKernelGGL(kern, dim3(batch), dim3(LANES * ldpc_dev::SPLIT_LANE_THREADS), lds, stream, a.llr,
+                       reinterpret_cast<uint4*>(a.msg), a.hard, a.result,
+                       reinterpret_cast<const ldpc_dev::circ_rec*>(a.circ), a.lay_off, a.steps, a.grp_off,
+                       a.groups, a.beta, a.alpha, a.active, p);
+    return hipGetLastError();
+}
+
+#define CUSAT_LDPC_SPLIT_INSTANTIATE(C, L) \
+    template hipError_t launch_ldpc_split_variant<C, L>(const ldpc_args&, int, hipStream_t);
+
+}  // namespace cusat

@@ -55,6 +55,8 @@ enum {
 /* Context flags */
 #define CUSAT_CTX_SEQUENTIAL_SCHEDULE 0x1u  /* one layer per step, as CUSAT_LDPC_SEQ=1
                                                (ldpc_decode_kernel.cu:1121-1125) */
+#define CUSAT_CTX_NO_SPLIT_KERNEL 0x2u      /* decode with the 6-wave kernel instead of the
+                                               12-wave split-edge kernel (same results) */
 
 typedef struct cusat_fec_ctx cusat_fec_ctx;
 
