@@ -359,6 +359,8 @@ __device__ __forceinline__ int decode_frame(unsigned char* lds_raw, const int16_
                 // keep the sign parity in one register (stops re-association
                 // of the XOR chain into every edge)
                 __asm__("" : "+v"(sg));
+                // the slot mask is wave-uniform (see ldpc_kernel_split.h)
+                const uint32_t slotu = __builtin_amdgcn_readfirstlane(slotm);
                 const u2 mn1p1 = as_u2(mn1) + (u2)1;
                 const uint32_t c1b = bits(as_s2u(c1)), c2b = bits(as_s2u(c2));
                 auto out = [&](s2 m, s2 x) {
@@ -376,7 +378,7 @@ __device__ __forceinline__ int decode_frame(unsigned char* lds_raw, const int16_
                         const s2 y = out(mv[e], xv[e]);
                         ys[e] = y;
                         const s2 d = y - ov[e];
-                        if ((slotm >> e) & 1u)
+                        if ((slotu >> e) & 1u)
                             lds_st(lds_raw, wa[e], perm(bits(d), sels[e]));
                         else
                             lds_st(lds_raw, wa[e], perm(bits(__builtin_elementwise_add_sat(av[e], d)), sels[e]));

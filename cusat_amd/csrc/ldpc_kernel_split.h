@@ -244,6 +244,10 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                     c2 = as_u2(a2);
                 }
                 __asm__("" : "+v"(sg));
+                // which edges write a delta slot: the same for every lane of the
+                // wave (circulant records are wave-uniform); taking it from the
+                // first active lane makes the per-edge choice a scalar branch
+                const uint32_t slotu = __builtin_amdgcn_readfirstlane(slotm);
                 // |x_e| == min1 exactly on the arg-min edge (all |x| >= min1)
                 const u2 mn1p1 = as_u2(mn1) + (u2)1;
                 const uint32_t c1b = bits(as_s2u(c1)), c2b = bits(as_s2u(c2));
@@ -262,7 +266,7 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                         const s2 y = out(mv[e], xv[e]);
                         ys[e] = y;
                         const s2 d = y - ov[e];
-                        if ((slotm >> e) & 1u)
+                        if ((slotu >> e) & 1u)
                             lds_st(lds_raw, wa[e], perm(bits(d), sels[e]));
                         else
                             lds_st(lds_raw, wa[e], perm(bits(__builtin_elementwise_add_sat(av[e], d)), sels[e]));
