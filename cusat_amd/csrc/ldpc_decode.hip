@@ -41,7 +41,8 @@ size_t ldpc_msg_bytes(int q, int cmax) { return (size_t)q * 180 * rec_words(cmax
 
 int ldpc_lds_bytes(int K, int R, int n_slots, int n_steps, int n_groups)
 {
-    return ((K + R) * 2 + n_slots * CIRC * 2 + 4 * (3 * n_steps + 1 + 8 * n_groups) + 15) & ~15;
+    // + 4 ints: the triple-buffered flag of wg_any
+    return ((K + R) * 2 + n_slots * CIRC * 2 + 4 * (3 * n_steps + 1 + 8 * n_groups + 4) + 15) & ~15;
 }
 
 bool ldpc_split_available(int cmax, bool exact, int lanes)

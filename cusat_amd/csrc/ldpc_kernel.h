@@ -139,6 +139,22 @@ struct kparams {
     int N, K, R, q, n_steps, max_trials, n_param_iters, n_slots, n_groups;
 };
 
+// Workgroup-wide OR of a per-thread flag with one barrier (replaces
+// __syncthreads_or, whose reduction scratch is static LDS).  flags: 3 ints of
+// LDS, zero before the first call; k: a call counter, the same in every
+// thread.  Call k writes slot k % 3 and clears slot (k + 1) % 3 -- last read
+// in call k - 2, which every thread finished before the barrier of call k - 1.
+__device__ __forceinline__ int wg_any(int v, int* flags, int& k)
+{
+    const int slot = k % 3;
+    if (v) flags[slot] = 1;
+    if (threadIdx.x == 0) flags[slot == 2 ? 0 : slot + 1] = 0;
+    __syncthreads();
+    const int r = flags[slot];
+    k++;
+    return r;
+}
+
 // Collision group record (int32 x 8, host-packed): LDS dword index of the
 // VN group's first pair dword, number of slots, the slots' dword indices.
 constexpr int GROUP_WORDS = 8;
@@ -167,6 +183,7 @@ __device__ __forceinline__ int decode_frame(unsigned char* lds_raw, const int16_
     int* sched_l = reinterpret_cast<int*>(dsl32 + p.n_slots * HALF);  // [2 n_steps]
     int* goff_l = sched_l + 2 * p.n_steps;                            // [n_steps + 1]
     int* grp_l = goff_l + p.n_steps + 1;                              // [n_groups][8]
+    int* flg = grp_l + GROUP_WORDS * p.n_groups;                      // [3] wg_any flags
     int16_t* app16 = reinterpret_cast<int16_t*>(lds_raw);
     int16_t* par16 = reinterpret_cast<int16_t*>(par32);
 
@@ -181,6 +198,8 @@ __device__ __forceinline__ int decode_frame(unsigned char* lds_raw, const int16_
     for (int k = t; k < 2 * p.n_steps; k += NTHR) sched_l[k] = steps[k];
     for (int k = t; k <= p.n_steps; k += NTHR) goff_l[k] = grp_off[k];
     for (int k = t; k < GROUP_WORDS * p.n_groups; k += NTHR) grp_l[k] = groups[k];
+    if (t < 3) flg[t] = 0;
+    int any_k = 0;
 
     // Load with the int16-APP input clamp (ldpc_decode_kernel.cu:655-670):
     // -32768 -> -32767.
@@ -245,7 +264,7 @@ __device__ __forceinline__ int decode_frame(unsigned char* lds_raw, const int16_
                 }
                 bad = (sg & 0x80008000u) != 0;
             }
-            if (__syncthreads_or(bad)) { ok = 0; break; }
+            if (wg_any(bad, flg, any_k)) { ok = 0; break; }
         }
         if (ok) { res = p.max_trials - it; break; }
         if (it == p.max_trials) break;

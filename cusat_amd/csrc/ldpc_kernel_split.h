@@ -56,6 +56,7 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
     int* sched_l = reinterpret_cast<int*>(xch + 3 * LANES * 2 * HALF);
     int* goff_l = sched_l + 2 * p.n_steps;
     int* grp_l = goff_l + p.n_steps + 1;
+    int* flg = grp_l + GROUP_WORDS * p.n_groups;  // [3] wg_any flags
     int16_t* app16 = reinterpret_cast<int16_t*>(lds_raw);
     int16_t* par16 = reinterpret_cast<int16_t*>(par32);
 
@@ -66,6 +67,7 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
     const int half = __builtin_amdgcn_readfirstlane(u >= LANE_THREADS ? 1 : 0);
     const int tt = u - half * LANE_THREADS;
     const int tt4 = 4 * tt;
+    const int tt4w = tt4 + 4 * HALF;  // byte offset of the pair dword after a wrap
     const bool act = tt < HALF;
     const int q = p.q;
     const int quarter = lane * 2 + half;  // which (lane, half) this wave group is
@@ -76,6 +78,8 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
     for (int k = t; k < 2 * p.n_steps; k += NTHR) sched_l[k] = steps[k];
     for (int k = t; k <= p.n_steps; k += NTHR) goff_l[k] = grp_off[k];
     for (int k = t; k < GROUP_WORDS * p.n_groups; k += NTHR) grp_l[k] = groups[k];
+    if (t < 3) flg[t] = 0;
+    int any_k = 0;
     // input clamp of the int16 APP (ldpc_decode_kernel.cu:655-670): -32768 -> -32767
     for (int v = t; v < p.K; v += NTHR) app16[vn_index(v)] = (int16_t)max((int)llr[v], -32767);
     for (int c = t; c < p.R; c += NTHR) {
@@ -131,7 +135,7 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                 }
                 bad = (sg & 0x80008000u) != 0;
             }
-            if (__syncthreads_or(bad)) { ok = 0; break; }
+            if (wg_any(bad, flg, any_k)) { ok = 0; break; }
         }
         if (ok) { res = p.max_trials - it; break; }
         if (it == p.max_trials) break;
@@ -163,7 +167,8 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
             const s2 zero = splat(0);
             const int ne = half ? NE1 : NE0;  // data edges of this half (wave-uniform)
             s2 xv[NE0], mv[NE0], av[NE0], ov[NE0];
-            int wa[NE0];
+            int ra_[NE0];  // LDS byte address of the edge's pair dword (its read and, outside
+                           // the collision slots, its write)
             uint32_t sels[NE0];
             uint32_t slotm = 0;
             s2 mn1 = splat(32767), mn2 = splat(32767);
@@ -175,16 +180,19 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                                        : (i > 0 ? (i - 1) * HALF + tt : (q - 1) * HALF + (tt > 0 ? tt - 1 : HALF - 1));
             const bool dummy = half == 1 && i == 0 && tt == 0;
             s2 xp = zero, mp = zero;
+            const circ_rec* cc = circ + lay_off[max(i, 0)] + half * NE0;
             if (work) {
-                const circ_rec* cc = circ + lay_off[i] + half * NE0;
 #pragma unroll
                 for (int e = 0; e < NE0; e++) {
                     if (e < ne) {
                         const circ_rec c = cc[e];
-                        int off;
-                        pair_addr(c, tt4, tt, off, sels[e]);
-                        const s2 ap = as_s2(perm(lds_ld(lds_raw, c.app_off + off), sels[e]));
-                        wa[e] = (c.slot_off < 0 ? c.app_off : c.slot_off) + off;
+                        // thread tt's pair dword wraps iff tt < sp; a wrap adds 180
+                        // dwords and swaps the halves
+                        const bool wrap = tt < c.sp;
+                        const int off = wrap ? tt4w : tt4;
+                        sels[e] = (wrap != (c.sel != SEL_ID)) ? SEL_ROT : SEL_ID;
+                        ra_[e] = c.app_off + off;
+                        const s2 ap = as_s2(perm(lds_ld(lds_raw, ra_[e]), sels[e]));
                         slotm |= (c.slot_off < 0 ? 0u : 1u) << e;
                         const s2 o = unpack_msg(rw, e);
                         const s2 x = __builtin_elementwise_sub_sat(ap, o);
@@ -250,11 +258,11 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                 const uint32_t slotu = __builtin_amdgcn_readfirstlane(slotm);
                 // |x_e| == min1 exactly on the arg-min edge (all |x| >= min1)
                 const u2 mn1p1 = as_u2(mn1) + (u2)1;
-                const uint32_t c1b = bits(as_s2u(c1)), c2b = bits(as_s2u(c2));
+                const u2 dc = c2 - c1;  // >= 0: the scale and clamp are monotone
                 auto out = [&](s2 m, s2 x) {
+                    // hit = 1 on the arg-min edge, else 0: magnitude c1 + hit (c2 - c1)
                     const u2 hit = __builtin_elementwise_sub_sat(mn1p1, as_u2(m));
-                    const uint32_t mask = bits(as_s2u((u2)0 - hit));
-                    const s2 mag = as_s2((mask & c2b) | (~mask & c1b));
+                    const s2 mag = as_s2u(c1 + hit * dc);
                     const s2 sgn = as_s2(sg ^ bits(x)) >> (short)15;
                     return (mag ^ sgn) - sgn;
                 };
@@ -266,10 +274,12 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                         const s2 y = out(mv[e], xv[e]);
                         ys[e] = y;
                         const s2 d = y - ov[e];
-                        if ((slotu >> e) & 1u)
-                            lds_st(lds_raw, wa[e], perm(bits(d), sels[e]));
-                        else
-                            lds_st(lds_raw, wa[e], perm(bits(__builtin_elementwise_add_sat(av[e], d)), sels[e]));
+                        if ((slotu >> e) & 1u) {
+                            const circ_rec c = cc[e];
+                            lds_st(lds_raw, ra_[e] + (c.slot_off - c.app_off), perm(bits(d), sels[e]));
+                        } else {
+                            lds_st(lds_raw, ra_[e], perm(bits(__builtin_elementwise_add_sat(av[e], d)), sels[e]));
+                        }
                     }
                 }
                 {
