@@ -15,8 +15,10 @@ n = round(64 alpha) clamped to 1..1023 (the decoder's odd-symmetric
 
     python -m cusat_amd.learn train --out cusat_amd/data/learned/s2_b6_16apsk.json
     python -m cusat_amd.learn eval  --params cusat_amd/data/learned/s2_b6_16apsk.json
+    python -m cusat_amd.learn train-set --codes s2x     # one file per MODCOD and frame size
 """
 import argparse
+import glob
 import json
 import os
 import sys
@@ -25,7 +27,46 @@ import time
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-DEFAULT_PARAMS = os.path.join(HERE, "data", "learned", "s2_b6_16apsk.json")
+LEARNED_DIR = os.path.join(HERE, "data", "learned")
+DEFAULT_PARAMS = os.path.join(LEARNED_DIR, "s2_b6_16apsk.json")
+
+# MODCODs of the mixed ACM streams with their quasi-error-free Es/N0
+# thresholds in dB (lib/cusat_dsp/acm_config.h:54-121): the twelve DVB-S2
+# MODCODs of the 12-MODCOD stream, then the three DVB-S2X ones.  No short
+# FECFRAME exists for rate 9/10 or for the S2X APSK codes used here.
+MODCODS12 = [("1/4", "QPSK", -0.05), ("1/2", "QPSK", 1.80), ("3/4", "QPSK", 4.43),
+             ("3/5", "8PSK", 5.80), ("9/10", "8PSK", 11.08), ("2/3", "16APSK", 9.17),
+             ("5/6", "16APSK", 11.71), ("3/4", "32APSK", 12.93), ("9/10", "32APSK", 16.05),
+             ("2/3", "QPSK", 3.50), ("3/4", "8PSK", 8.11), ("3/4", "16APSK", 10.41)]
+S2X = [("4/5", "64APSK", 16.10), ("135/180", "128APSK", 18.00), ("135/180", "256APSK", 23.34)]
+NO_SHORT = {"9/10", "135/180"}
+NO_SHORT_MOD = {"64APSK", "128APSK", "256APSK"}
+
+
+def has_short(rate, mod):
+    return rate not in NO_SHORT and mod not in NO_SHORT_MOD
+
+
+def params_name(fs, rate, mod):
+    return "%s_%s_%s.json" % ("normal" if fs == 0 else "short", mod.lower(), rate.replace("/", "-"))
+
+
+def _code_of(path):
+    with open(path) as fh:
+        d = json.load(fh)
+    return d.get("framesize"), d.get("rate"), d.get("modulation")
+
+
+def params_for(fs, rate, mod, directory=LEARNED_DIR):
+    """Path of the trained parameters of (frame size, rate, modulation) in
+    `directory`, or None; a file whose own evaluation found no gain
+    ("use": false) is skipped."""
+    for f in sorted(glob.glob(os.path.join(directory, "*.json"))):
+        with open(f) as fh:
+            d = json.load(fh)
+        if (d.get("framesize"), d.get("rate"), d.get("modulation")) == (fs, rate, mod) and d.get("use", True):
+            return f
+    return None
 
 
 def quantize(alpha, beta):
@@ -129,14 +170,20 @@ def train(framesize=0, rate="2/3", modulation="16APSK", n_iters=20, batch=512, s
 
 
 def evaluate(params_path=DEFAULT_PARAMS, points=(8.8, 9.0, 9.2, 9.4, 9.6), frames=16384, chunk=1024,
-             learned_trials=20, plain_trials=50, pool_frames=2048, seed=777, log=print):
+             learned_trials=20, plain_trials=50, pool_frames=2048, seed=777, log=print, params=None):
     """FER and decoded info Gbit/s per Es/N0 of: the plain decoder (T = 50),
     the learned decoder alone (T = learned_trials) and the learned decoder
-    with plain fallback, on the same noisy LLRs."""
+    with plain fallback, on the same noisy LLRs.  `params`: a parameter dict
+    as train() returns, instead of the file."""
     import torch
     import cusat_amd as ca
-    d, bq, aq = load_params(params_path)
+    if params is None:
+        d, bq, aq = load_params(params_path)
+    else:
+        d = params
+        bq, aq = np.array(d["beta_q"], np.int16), np.array(d["alpha_q"], np.int16)
     fs, rate, mod = d["framesize"], d["rate"], d["modulation"]
+    chunk = min(chunk, frames)
     ctx = ca.FecContext(fs, rate, mod, max_trials=plain_trials, max_batch=chunk)
     ctx.set_params(bq, aq)
     # the same structure (learned_trials + plain fallback) with the plain
@@ -202,6 +249,50 @@ def evaluate(params_path=DEFAULT_PARAMS, points=(8.8, 9.0, 9.2, 9.4, 9.6), frame
     return rows
 
 
+def train_set(codes="s2x", steps=200, batch=512, pool_frames=1024, out_dir=LEARNED_DIR, margin=0.3,
+              eval_frames=2048, force=False, only=None, log=print):
+    """Train one parameter file per (frame size, MODCOD) of a mixed stream
+    (codes: "mixed12" or "s2x"), each over Es/N0 in [threshold - 0.2,
+    threshold + margin + 0.2], then evaluate it at threshold + margin
+    against the plain decoder on the same LLRs.  The file records the
+    evaluation and "use": the learned decoder with plain fallback decoded
+    no more frames wrongly than the plain one, in fewer mean iterations."""
+    mods = MODCODS12 + (S2X if codes == "s2x" else [])
+    if only:
+        mods = [m for m in mods if m[1] in only]
+    os.makedirs(out_dir, exist_ok=True)
+    done = []
+    for rate, mod, th in mods:
+        for fs in ((0, 1) if has_short(rate, mod) else (0,)):
+            path = os.path.join(out_dir, params_name(fs, rate, mod))
+            have = [f for f in glob.glob(os.path.join(out_dir, "*.json"))
+                    if _code_of(f) == (fs, rate, mod)]
+            if have and not force:
+                log("skip %s %s %s: %s" % (fs, mod, rate, os.path.basename(have[0])))
+                continue
+            t0 = time.time()
+            log("train %s %s %s: Es/N0 %.2f..%.2f dB" % ("normal" if fs == 0 else "short", mod, rate,
+                                                         th - 0.2, th + margin + 0.2))
+            d = train(fs, rate, mod, batch=batch, steps=steps, es_lo=th - 0.2, es_hi=th + margin + 0.2,
+                      pool_frames=pool_frames, log=lambda s: None)
+            rows = evaluate(points=(th + margin,), frames=eval_frames, pool_frames=pool_frames,
+                            log=lambda s: None, params=d)
+            r = {x["decoder"]: x for x in rows}
+            p, lf = r["plain"], r["learned+fallback"]
+            d["eval"] = {"es_n0_db": th + margin, "frames": eval_frames,
+                         "plain": {k: p[k] for k in ("fer", "mean_iterations", "gbps_info")},
+                         "learned+fallback": {k: lf[k] for k in ("fer", "mean_iterations", "gbps_info",
+                                                                  "fallback_rate")}}
+            d["use"] = bool(lf["fer"] <= p["fer"] and lf["mean_iterations"] < p["mean_iterations"])
+            with open(path, "w") as f:
+                json.dump(d, f, indent=1)
+            log("  %.1f s: plain FER %.2e iters %.2f | learned+fallback FER %.2e iters %.2f fallback %.3f"
+                " -> use %s" % (time.time() - t0, p["fer"], p["mean_iterations"], lf["fer"],
+                                lf["mean_iterations"], lf["fallback_rate"], d["use"]))
+            done.append(path)
+    return done
+
+
 def fer_crossing(rows, decoder, target=1e-3):
     """Es/N0 where the FER of `decoder` falls through `target` (log-linear
     interpolation between the sweep points) and the Gbit/s there."""
@@ -228,6 +319,15 @@ def main(argv=None):
     t.add_argument("--lr-beta", type=float, default=0.02)
     t.add_argument("--float-model", action="store_true", help="train the fp32 model without rounding")
     t.add_argument("--es", default="8.9,9.6", help="training Es/N0 range in dB")
+    ts = sub.add_parser("train-set")
+    ts.add_argument("--codes", default="s2x", choices=["mixed12", "s2x"])
+    ts.add_argument("--steps", type=int, default=200)
+    ts.add_argument("--batch", type=int, default=512)
+    ts.add_argument("--pool", type=int, default=1024)
+    ts.add_argument("--eval-frames", type=int, default=2048)
+    ts.add_argument("--out-dir", default=LEARNED_DIR)
+    ts.add_argument("--only", default=None, help="comma-separated modulations to train")
+    ts.add_argument("--force", action="store_true", help="retrain codes that have a file")
     e = sub.add_parser("eval")
     e.add_argument("--params", default=DEFAULT_PARAMS)
     e.add_argument("--frames", type=int, default=16384)
@@ -243,6 +343,9 @@ def main(argv=None):
         with open(a.out, "w") as f:
             json.dump(d, f, indent=1)
         log("wrote " + a.out)
+    elif a.cmd == "train-set":
+        train_set(a.codes, a.steps, a.batch, a.pool, a.out_dir, eval_frames=a.eval_frames, force=a.force,
+                  only=a.only.split(",") if a.only else None, log=log)
     else:
         rows = evaluate(a.params, [float(x) for x in a.points.split(",")], a.frames, log=log)
         summary = {m: fer_crossing(rows, m)

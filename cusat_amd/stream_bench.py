@@ -10,9 +10,11 @@ used for short frames):
   8PSK 3/4, 16APSK 3/4 to make twelve), 30% short FECFRAMEs where the MODCOD
   has one (no 9/10 short code), threshold + 0.5 dB, plain decoder.
 * s2x: 4096 frames over 64APSK 4/5, 128APSK 3/4, 256APSK 3/4 and the twelve
-  above, 25% short (DVB-S2 MODCODs only), threshold + 0.3 dB; codes with
-  trained parameters (cusat_amd/data/learned) run the learned decoder with
-  in-kernel plain fallback, against the all-plain stream on the same LLRs.
+  above, 25% short (DVB-S2 MODCODs only), threshold + 0.3 dB; every
+  (frame size, MODCOD) with trained parameters (cusat_amd/data/learned, one
+  file per code from `python -m cusat_amd.learn train-set`) runs the learned
+  decoder with in-kernel plain fallback, against the all-plain stream on the
+  same LLRs.
 
     python -m cusat_amd.stream_bench --workload mixed12
 """
@@ -24,13 +26,7 @@ import time
 
 import numpy as np
 
-MODCODS12 = [("1/4", "QPSK", -0.05), ("1/2", "QPSK", 1.80), ("3/4", "QPSK", 4.43),
-             ("3/5", "8PSK", 5.80), ("9/10", "8PSK", 11.08), ("2/3", "16APSK", 9.17),
-             ("5/6", "16APSK", 11.71), ("3/4", "32APSK", 12.93), ("9/10", "32APSK", 16.05),
-             ("2/3", "QPSK", 3.50), ("3/4", "8PSK", 8.11), ("3/4", "16APSK", 10.41)]
-S2X = [("4/5", "64APSK", 16.10), ("135/180", "128APSK", 18.00), ("135/180", "256APSK", 23.34)]
-NO_SHORT = {"9/10", "135/180"}
-NO_SHORT_MOD = {"64APSK", "128APSK", "256APSK"}
+from cusat_amd.learn import MODCODS12, NO_SHORT, NO_SHORT_MOD, S2X
 
 
 def snr_scales(es):
@@ -82,12 +78,14 @@ def run(workload="mixed12", n_frames=None, seed=20240612, reps=5, learned=None, 
     torch.cuda.synchronize()
     log("generated %d frames over %d (frame, MODCOD) groups in %.1f s" % (n_frames, len(keys), time.time() - t0))
 
-    # learned parameters where trained (16APSK 2/3 normal)
+    # learned parameters of every (frame size, MODCOD) that has a trained
+    # file whose own evaluation found a gain
     have_learned = set()
-    if learned and os.path.exists(learn.DEFAULT_PARAMS):
-        d, bq, aq = learn.load_params()
+    if learned:
         for k, c in ctxs.items():
-            if (k[0], k[1], k[2]) == (d["framesize"], d["rate"], d["modulation"]):
+            path = learn.params_for(k[0], k[1], k[2])
+            if path:
+                _, bq, aq = learn.load_params(path)
                 c.set_params(bq, aq)
                 have_learned.add(k)
 
@@ -184,7 +182,12 @@ def run(workload="mixed12", n_frames=None, seed=20240612, reps=5, learned=None, 
             same = False
         kbb = ctxs[k].info["k_bch"] // 8
         ok[n] = tn[n] >= 0 and vn[n] == 1 and np.array_equal(hn[o:o + kbb], info_g[k][j])
-        iters[n] = (50 - tn[n]) if tn[n] >= 0 else 50
+        plain_it = (50 - tn[n]) if tn[n] >= 0 else 50
+        if learned and k in have_learned:
+            # learned pass of 20 iterations, then the plain re-decode
+            iters[n] = 20 + plain_it if fn_[n] == 1 else 20 - tn[n]
+        else:
+            iters[n] = plain_it
     res = {
         "workload": workload, "frames": n_frames, "modcods": len(set((k[1], k[2]) for k in keys)),
         "groups": len(keys), "short_fraction": float(np.mean([s[0] == 1 for s in spec])),
@@ -195,7 +198,8 @@ def run(workload="mixed12", n_frames=None, seed=20240612, reps=5, learned=None, 
         "gbps_info_stream": round(kb_total / (ms_graph * 1e-3) / 1e9, 3),
         "gbps_info_per_modcod": round(kb_total / (ms_groups * 1e-3) / 1e9, 3),
         "speedup": round(ms_groups / ms_graph, 3), "tuning": chosen,
-        "fer": float(1 - ok.mean()), "mean_iterations_reported": float(iters.mean()),
+        "fer": float(1 - ok.mean()), "mean_iterations": float(iters.mean()),
+        "fallback_rate": float(fn_.mean()) if learned else 0.0,
         "identical_to_per_modcod": bool(same),
     }
     log(json.dumps(res))

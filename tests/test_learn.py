@@ -116,3 +116,33 @@ def test_quantisation():
     assert b.dtype == np.int16 and a.dtype == np.int16
     assert list(a[0]) == [32, 64, 81, 1023]
     assert list(b[0]) == [0, 0, 2, 255]
+
+
+def test_trained_parameter_files():
+    """Every shipped parameter file fits its code (n_iters x q int16 tables
+    in the decoder hooks' ranges), the lookup finds exactly the files whose
+    own evaluation found a gain, and no two files claim one code."""
+    import glob
+    import json
+    import os
+    seen = set()
+    for f in sorted(glob.glob(os.path.join(learn.LEARNED_DIR, "*.json"))):
+        d = json.load(open(f))
+        code = (d["framesize"], d["rate"], d["modulation"])
+        assert code not in seen, f
+        seen.add(code)
+        q = ca.geometry(*code)["q"]
+        b, a = np.array(d["beta_q"]), np.array(d["alpha_q"])
+        assert b.shape == a.shape == (d["n_iters"], q), f
+        assert b.min() >= 0 and b.max() <= 255 and a.min() >= 1 and a.max() <= 1023, f
+        # the stored fp32 values are rounded to 5 decimals: a value on a
+        # rounding boundary may quantise one step away
+        bq, aq = learn.quantize(d["alpha"], d["beta"])
+        assert np.abs(bq.astype(int) - b).max() <= 1 and np.mean(bq != b) < 0.01, f
+        assert np.abs(aq.astype(int) - a).max() <= 1 and np.mean(aq != a) < 0.01, f
+        found = learn.params_for(*code)
+        assert (found == f) == d.get("use", True), f
+    # every code of the s2x stream has been trained
+    want = {(fs, r, m) for r, m, _ in learn.MODCODS12 + learn.S2X
+            for fs in ((0, 1) if learn.has_short(r, m) else (0,))}
+    assert want <= seen
