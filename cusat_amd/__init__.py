@@ -211,10 +211,11 @@ STREAM_FRAME = np.dtype([("code", np.int32), ("reserved", np.int32), ("llr_offse
 
 class StreamTuning(ctypes.Structure):
     _fields_ = [("wg_per_cu", ctypes.c_int), ("frames_per_job", ctypes.c_int), ("order", ctypes.c_int),
-                ("reserved", ctypes.c_int)]
+                ("split_edges", ctypes.c_int)]
 
     def as_dict(self):
-        return {"wg_per_cu": self.wg_per_cu, "frames_per_job": self.frames_per_job, "order": self.order}
+        return {"wg_per_cu": self.wg_per_cu, "frames_per_job": self.frames_per_job, "order": self.order,
+                "split_edges": self.split_edges}
 
 
 class StreamContext:
@@ -240,7 +241,8 @@ class StreamContext:
         frames = np.ascontiguousarray(frames, STREAM_FRAME)
         t = None
         if tuning is not None:
-            t = StreamTuning(tuning["wg_per_cu"], tuning["frames_per_job"], tuning["order"], 0)
+            t = StreamTuning(tuning["wg_per_cu"], tuning["frames_per_job"], tuning["order"],
+                             tuning.get("split_edges", 0))
         _check(lib().cusat_stream_plan(self._h, _ptr(frames), len(frames),
                                        ctypes.byref(t) if t is not None else None), "cusat_stream_plan")
 

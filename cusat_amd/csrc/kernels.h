@@ -40,12 +40,12 @@ bool ldpc_select_variant(int cnt_min, int cnt_max, int lanes, int* cmax, bool* e
 // message record bytes per frame for a variant
 size_t ldpc_msg_bytes(int q, int cmax);
 hipError_t launch_ldpc_decode(const ldpc_args& a, int batch, hipStream_t stream);
-// split-edge kernel: exists for exact-degree codes; its LDS adds the partial
+// split-edge kernel: exists for every variant; its LDS adds the partial
 // exchange area, its message records are [layer][half][pair]
 bool ldpc_split_available(int cmax, bool exact, int lanes);
 int ldpc_lds_bytes_split(int K, int R, int n_slots, int n_steps, int n_groups, int lanes);
 size_t ldpc_msg_bytes_split(int q, int cmax);
-template <int CMAX, int LANES>
+template <int CMAX, bool EXACT, int LANES>
 hipError_t launch_ldpc_split_variant(const ldpc_args& a, int batch, hipStream_t stream);
 template <int CMAX, bool EXACT, int LANES>
 hipError_t launch_ldpc_variant(const ldpc_args& a, int batch, hipStream_t stream);

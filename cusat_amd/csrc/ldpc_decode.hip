@@ -47,9 +47,11 @@ int ldpc_lds_bytes(int K, int R, int n_slots, int n_steps, int n_groups)
 
 bool ldpc_split_available(int cmax, bool exact, int lanes)
 {
-    if (!exact) return false;
-    const int* b = lanes == 2 ? EXACT2 : EXACT1;
-    const int n = lanes == 2 ? (int)(sizeof(EXACT2) / sizeof(int)) : (int)(sizeof(EXACT1) / sizeof(int));
+    // every variant of the 6-wave kernel has a split-edge form
+    const int* b = !exact ? GUARDED : lanes == 2 ? EXACT2 : EXACT1;
+    const int n = !exact        ? (int)(sizeof(GUARDED) / sizeof(int))
+                  : lanes == 2 ? (int)(sizeof(EXACT2) / sizeof(int))
+                               : (int)(sizeof(EXACT1) / sizeof(int));
     for (int k = 0; k < n; k++)
         if (b[k] == cmax) return true;
     return false;
@@ -70,11 +72,15 @@ size_t ldpc_msg_bytes_split(int q, int cmax)
 hipError_t launch_ldpc_decode(const ldpc_args& a, int batch, hipStream_t stream)
 {
     if (a.split && ldpc_split_available(a.cmax, a.exact, a.lanes)) {
-#define SCASE(C, L) \
-    if (a.cmax == C && a.lanes == L) return launch_ldpc_split_variant<C, L>(a, batch, stream);
-        SCASE(2, 2) SCASE(3, 2) SCASE(4, 2) SCASE(5, 2) SCASE(7, 2) SCASE(8, 2) SCASE(9, 2) SCASE(11, 2)
-        SCASE(12, 2) SCASE(16, 2) SCASE(20, 2) SCASE(25, 2) SCASE(28, 2)
-        SCASE(2, 1) SCASE(3, 1) SCASE(4, 1) SCASE(8, 1) SCASE(9, 1) SCASE(11, 1) SCASE(25, 1)
+#define SCASE(C, E, L) \
+    if (a.cmax == C && a.exact == E && a.lanes == L) return launch_ldpc_split_variant<C, E, L>(a, batch, stream);
+        SCASE(2, true, 2) SCASE(3, true, 2) SCASE(4, true, 2) SCASE(5, true, 2) SCASE(7, true, 2)
+        SCASE(8, true, 2) SCASE(9, true, 2) SCASE(11, true, 2) SCASE(12, true, 2) SCASE(16, true, 2)
+        SCASE(20, true, 2) SCASE(25, true, 2) SCASE(28, true, 2)
+        SCASE(2, true, 1) SCASE(3, true, 1) SCASE(4, true, 1) SCASE(8, true, 1) SCASE(9, true, 1)
+        SCASE(11, true, 1) SCASE(25, true, 1)
+        SCASE(6, false, 2) SCASE(12, false, 2) SCASE(18, false, 2) SCASE(24, false, 2) SCASE(30, false, 2)
+        SCASE(6, false, 1) SCASE(12, false, 1) SCASE(18, false, 1) SCASE(24, false, 1) SCASE(30, false, 1)
 #undef SCASE
     }
 #define CASE(C, E, L) \

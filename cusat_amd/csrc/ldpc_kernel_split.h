@@ -5,9 +5,11 @@
 // SIMD) instead of 6 (2, 2, 1, 1):
 //
 //  * A lane (layer of the schedule step) is 6 waves: waves 0-2 ("half 0")
-//    evaluate data edges 0 .. NE0-1 and the parity edge P(i, j) of the check
-//    pairs (t, t + 180), waves 3-5 ("half 1") data edges NE0 .. CMAX-1 and
-//    the second parity edge.  Every wave still walks one circulant at a time,
+//    evaluate data edges 0 .. n0-1 and the parity edge P(i, j) of the check
+//    pairs (t, t + 180), waves 3-5 ("half 1") data edges n0 .. cnt-1 and the
+//    second parity edge, where cnt is the layer's number of data circulants
+//    (CMAX for an exact-degree code, lay_cnt[i] <= CMAX otherwise) and
+//    n0 = ceil(cnt / 2).  Every wave still walks one circulant at a time,
 //    so circulant records stay scalar loads.
 //  * Each half reduces its edges to (min1, min2, sign) of |x| -- the offset
 //    beta is applied once per check, max(min|x| - beta, 0) being the minimum
@@ -38,11 +40,11 @@ struct rec_half_t {
     static constexpr int V4 = WORDS / 4;
 };
 
-template <int CMAX, int LANES>
+template <int CMAX, bool EXACT, int LANES>
 __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const int16_t* __restrict__ llr,
                                                   uint4* __restrict__ rec_base, uint8_t* __restrict__ h,
                                                   const circ_rec* __restrict__ circ,
-                                                  const int* __restrict__ lay_off,
+                                                  const int* __restrict__ lay_cnt, const int* __restrict__ lay_off,
                                                   const int* __restrict__ steps, const int* __restrict__ grp_off,
                                                   const int* __restrict__ groups, const int* __restrict__ beta,
                                                   const int* __restrict__ alpha, const kparams& p)
@@ -125,13 +127,16 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                     sg ^= tt > 0 ? w : (w << 16);
                 }
                 const circ_rec* cc = circ + lay_off[i];
+                const int cnt = EXACT ? CMAX : lay_cnt[i];
 #pragma unroll
                 for (int e = 0; e < CMAX; e++) {
-                    const circ_rec c = cc[e];
-                    int off;
-                    uint32_t sel;
-                    pair_addr(c, tt4, tt, off, sel);
-                    sg ^= perm(lds_ld(lds_raw, c.app_off + off), sel);
+                    if (EXACT || e < cnt) {
+                        const circ_rec c = cc[e];
+                        int off;
+                        uint32_t sel;
+                        pair_addr(c, tt4, tt, off, sel);
+                        sg ^= perm(lds_ld(lds_raw, c.app_off + off), sel);
+                    }
                 }
                 bad = (sg & 0x80008000u) != 0;
             }
@@ -165,7 +170,10 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
             }
             const bool work = act && i >= 0;
             const s2 zero = splat(0);
-            const int ne = half ? NE1 : NE0;  // data edges of this half (wave-uniform)
+            // data edges of the layer and of this half (wave-uniform)
+            const int cnt = EXACT ? CMAX : __builtin_amdgcn_readfirstlane(lay_cnt[max(i, 0)]);
+            const int n0 = EXACT ? NE0 : (cnt + 1) >> 1;
+            const int ne = half ? cnt - n0 : n0;
             s2 xv[NE0], mv[NE0], av[NE0], ov[NE0];
             int ra_[NE0];  // LDS byte address of the edge's pair dword (its read and, outside
                            // the collision slots, its write)
@@ -180,7 +188,7 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                                        : (i > 0 ? (i - 1) * HALF + tt : (q - 1) * HALF + (tt > 0 ? tt - 1 : HALF - 1));
             const bool dummy = half == 1 && i == 0 && tt == 0;
             s2 xp = zero, mp = zero;
-            const circ_rec* cc = circ + lay_off[max(i, 0)] + half * NE0;
+            const circ_rec* cc = circ + lay_off[max(i, 0)] + half * n0;
             if (work) {
 #pragma unroll
                 for (int e = 0; e < NE0; e++) {
@@ -336,11 +344,12 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
     return res;
 }
 
-template <int CMAX, int LANES>
+template <int CMAX, bool EXACT, int LANES>
 __global__ void __launch_bounds__(LANES* SPLIT_LANE_THREADS)
     ldpc_kernel_split(const int16_t* __restrict__ llr_all, uint4* __restrict__ msg_all,
                       uint8_t* __restrict__ hard_all, int32_t* __restrict__ result,
-                      const circ_rec* __restrict__ circ, const int* __restrict__ lay_off,
+                      const circ_rec* __restrict__ circ, const int* __restrict__ lay_cnt,
+                      const int* __restrict__ lay_off,
                       const int* __restrict__ steps, const int* __restrict__ grp_off,
                       const int* __restrict__ groups, const int* __restrict__ beta,
                       const int* __restrict__ alpha, const int32_t* __restrict__ active, kparams p)
@@ -348,18 +357,18 @@ __global__ void __launch_bounds__(LANES* SPLIT_LANE_THREADS)
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int cw = blockIdx.x;
     if (active && active[cw] == 0) return;
-    const int res = decode_frame_split<CMAX, LANES>(
+    const int res = decode_frame_split<CMAX, EXACT, LANES>(
         lds_raw, llr_all + (size_t)cw * p.N, msg_all + (size_t)cw * p.q * 2 * HALF * rec_half_t<CMAX>::V4,
-        hard_all + (size_t)cw * (p.K >> 3), circ, lay_off, steps, grp_off, groups, beta, alpha, p);
+        hard_all + (size_t)cw * (p.K >> 3), circ, lay_cnt, lay_off, steps, grp_off, groups, beta, alpha, p);
     if (threadIdx.x == 0) result[cw] = res;
 }
 
 }  // namespace ldpc_dev
 
-template <int CMAX, int LANES>
+template <int CMAX, bool EXACT, int LANES>
 hipError_t launch_ldpc_split_variant(const ldpc_args& a, int batch, hipStream_t stream)
 {
-    auto kern = &ldpc_dev::ldpc_kernel_split<CMAX, LANES>;
+    auto kern = &ldpc_dev::ldpc_kernel_split<CMAX, EXACT, LANES>;
     const int lds = ldpc_lds_bytes_split(a.K, a.R, a.n_slots, a.n_steps, a.n_groups, LANES);
     static int configured = 0;
     if (lds > __atomic_load_n(&configured, __ATOMIC_ACQUIRE)) {
@@ -372,12 +381,12 @@ hipError_t launch_ldpc_split_variant(const ldpc_args& a, int batch, hipStream_t 
                               a.n_slots, a.n_groups};
     hipLaunchKernelGGL(kern, dim3(batch), dim3(LANES * ldpc_dev::SPLIT_LANE_THREADS), lds, stream, a.llr,
                        reinterpret_cast<uint4*>(a.msg), a.hard, a.result,
-                       reinterpret_cast<const ldpc_dev::circ_rec*>(a.circ), a.lay_off, a.steps, a.grp_off,
+                       reinterpret_cast<const ldpc_dev::circ_rec*>(a.circ), a.lay_cnt, a.lay_off, a.steps, a.grp_off,
                        a.groups, a.beta, a.alpha, a.active, p);
     return hipGetLastError();
 }
 
-#define CUSAT_LDPC_SPLIT_INSTANTIATE(C, L) \
-    template hipError_t launch_ldpc_split_variant<C, L>(const ldpc_args&, int, hipStream_t);
+#define CUSAT_LDPC_SPLIT_INSTANTIATE(C, E, L) \
+    template hipError_t launch_ldpc_split_variant<C, E, L>(const ldpc_args&, int, hipStream_t);
 
 }  // namespace cusat

@@ -1,12 +1,15 @@
 // Persistent decoder of a mixed ACM stream: frames of several codes (normal
 // and short FECFRAMEs, any MODCOD of the contexts given) in ONE launch.
 //
-// A fixed grid of workgroups (384 threads, one frame at a time each) takes
-// jobs from an atomic queue: a job is a run of frames (same code, consecutive
-// in the plan's order).  Per frame the workgroup
+// A fixed grid of workgroups (one frame at a time each) takes jobs from an
+// atomic queue: a job is a run of frames (same code, consecutive in the
+// plan's order).  Two builds of the kernel: 6-wave workgroups running
+// decode_frame, and 12-wave workgroups running decode_frame_split (the edges
+// of a check over two wave groups); the tuning picks one.
+// Per frame the workgroup
 //   1. decodes with the code's decoder variant (a switch over the compiled
-//      decode_frame instantiations; one-lane short-frame schedules run with
-//      the second lane idle, exactly as the per-code decoder),
+//      instantiations; one-lane short-frame schedules run with the second
+//      lane idle, exactly as the per-code decoder),
 //   2. checks the BCH syndrome of its hard decision with one wave
 //      (bch_check.hip's algorithm, byte table staged in the then free LDS),
 //   3. in learned mode (codes with parameters): a frame that fails
@@ -31,12 +34,18 @@
 #include "cusat_fec.h"
 #include "kernels.h"
 #include "ldpc_kernel.h"
+#include "ldpc_kernel_split.h"
 
 using namespace cusat;
 
 namespace {
 
-constexpr int STREAM_THREADS = 2 * ldpc_dev::LANE_THREADS;
+// workgroup size of the 6-wave (WIDE = false) and 12-wave (WIDE = true) kernel
+template <bool WIDE>
+constexpr int stream_threads()
+{
+    return WIDE ? 2 * ldpc_dev::SPLIT_LANE_THREADS : 2 * ldpc_dev::LANE_THREADS;
+}
 
 struct code_desc {
     const ldpc_dev::circ_rec* circ;
@@ -51,7 +60,8 @@ struct code_desc {
     const uint64_t* bch_pw;
     int N, K, R, q, n_steps, n_slots, n_groups;
     int max_trials, learned_trials, n_param_iters;
-    int variant;
+    int variant;        // decode_frame variant (CUSAT_STREAM_VARIANTS id)
+    int variant_split;  // decode_frame_split variant (CUSAT_STREAM_SPLIT_VARIANTS id), or -1
     int bch_bytes, bch_degree, bch_chunk, bch_words;
 };
 
@@ -62,6 +72,10 @@ struct code_desc {
     X(12, 28, true) X(13, 6, false) X(14, 12, false) X(15, 18, false) X(16, 24, false)              \
     X(17, 30, false)
 
+// two-lane split-edge variants of the 12-wave kernel: (id, CMAX, exact), the
+// same set (every code the stream takes has one)
+#define CUSAT_STREAM_SPLIT_VARIANTS(X) CUSAT_STREAM_VARIANTS(X)
+
 int variant_id(int cmax, bool exact)
 {
 #define X(ID, C, E) \
@@ -71,13 +85,36 @@ int variant_id(int cmax, bool exact)
     return -1;
 }
 
-__device__ __forceinline__ int decode_variant(int v, unsigned char* lds, const int16_t* llr, uint4* rec,
-                                              uint8_t* h, const code_desc& c, const int* beta,
-                                              const int* alpha, const ldpc_dev::kparams& p)
+int split_variant_id(int cmax, bool exact)
 {
-    switch (v) {
-#define X(ID, C, E)                                                                                   \
-    case ID:                                                                                          \
+    if (!ldpc_split_available(cmax, exact, 2)) return -1;
+#define X(ID, C, E) \
+    if (cmax == C && exact == E) return ID;
+    CUSAT_STREAM_SPLIT_VARIANTS(X)
+#undef X
+    return -1;
+}
+
+// 12-wave workgroups: the split-edge decoder; 6-wave: decode_frame
+template <bool WIDE>
+__device__ __forceinline__ int decode_variant(const code_desc& c, unsigned char* lds, const int16_t* llr,
+                                              uint4* rec, uint8_t* h, const int* beta, const int* alpha,
+                                              const ldpc_dev::kparams& p)
+{
+    if (WIDE) {
+        switch (c.variant_split) {
+#define X(ID, C, E)                                                                                         \
+    case ID:                                                                                                \
+        return ldpc_dev::decode_frame_split<C, E, 2>(lds, llr, rec, h, c.circ, c.lay_cnt, c.lay_off, c.steps, \
+                                                     c.grp_off, c.groups, beta, alpha, p);
+            CUSAT_STREAM_SPLIT_VARIANTS(X)
+#undef X
+        }
+        return -1;
+    }
+    switch (c.variant) {
+#define X(ID, C, E)                                                                                      \
+    case ID:                                                                                             \
         return ldpc_dev::decode_frame<C, E, 2>(lds, llr, rec, h, c.circ, c.lay_cnt, c.lay_off, c.steps, \
                                                c.grp_off, c.groups, beta, alpha, p);
         CUSAT_STREAM_VARIANTS(X)
@@ -133,12 +170,13 @@ __device__ __forceinline__ int bch_wave(const uint8_t* __restrict__ src, const u
 
 // Whole workgroup: BCH check of the hard bytes at h (written by this
 // workgroup before the call); returns 1 = valid, the same value in every thread.
+template <int NT>
 __device__ __forceinline__ int bch_frame(unsigned char* lds, int* flag, const uint8_t* __restrict__ h,
                                          const code_desc& c)
 {
     uint64_t* tab = reinterpret_cast<uint64_t*>(lds);
     __syncthreads();  // the decoder is done with the LDS; h is written
-    for (int k = threadIdx.x; k < 256 * c.bch_words; k += STREAM_THREADS) tab[k] = c.bch_lut[k];
+    for (int k = threadIdx.x; k < 256 * c.bch_words; k += NT) tab[k] = c.bch_lut[k];
     __syncthreads();
     if (threadIdx.x < 64) {
         int ok = 0;
@@ -156,15 +194,20 @@ __device__ __forceinline__ int bch_frame(unsigned char* lds, int* flag, const ui
     return ok;
 }
 
-__global__ void __launch_bounds__(STREAM_THREADS)
+// ctl_off: byte offset in the dynamic LDS of the two control ints (queue
+// slot, BCH flag), after the decoders' own LDS
+template <bool WIDE>
+__global__ void __launch_bounds__(stream_threads<WIDE>())
     ldpc_stream_kernel(const code_desc* __restrict__ codes, const cusat_stream_frame* __restrict__ frames,
                        const int2* __restrict__ jobs, const int* __restrict__ order, int n_jobs,
                        unsigned* counters, const int16_t* __restrict__ llr, uint8_t* __restrict__ hard,
                        int32_t* __restrict__ trials, int32_t* __restrict__ valid,
-                       int32_t* __restrict__ fallback, uint4* __restrict__ ws, size_t ws_v4, int learned)
+                       int32_t* __restrict__ fallback, uint4* __restrict__ ws, size_t ws_v4, int learned,
+                       int ctl_off)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    __shared__ int job_s, flag_s;
+    int& job_s = reinterpret_cast<int*>(lds_raw + ctl_off)[0];
+    int& flag_s = reinterpret_cast<int*>(lds_raw + ctl_off)[1];
     uint4* rec = ws + (size_t)blockIdx.x * ws_v4;
     for (;;) {
         if (threadIdx.x == 0) job_s = (int)atomicAdd(&counters[0], 1u);
@@ -186,9 +229,9 @@ __global__ void __launch_bounds__(STREAM_THREADS)
                 const ldpc_dev::kparams p{c.N, c.K, c.R, c.q, c.n_steps,
                                           lp ? c.learned_trials : c.max_trials,
                                           lp ? c.n_param_iters : 0, c.n_slots, c.n_groups};
-                res = decode_variant(c.variant, lds_raw, fl, rec, fh, c, lp ? c.beta : nullptr,
-                                     lp ? c.alpha : nullptr, p);
-                ok = bch_frame(lds_raw, &flag_s, fh, c);
+                res = decode_variant<WIDE>(c, lds_raw, fl, rec, fh, lp ? c.beta : nullptr,
+                                           lp ? c.alpha : nullptr, p);
+                ok = bch_frame<stream_threads<WIDE>()>(lds_raw, &flag_s, fh, c);
                 if (!lp || (res >= 0 && ok)) break;
                 fb = 1;
             }
@@ -222,10 +265,13 @@ __global__ void __launch_bounds__(STREAM_THREADS)
 
 struct cusat_stream_ctx {
     std::vector<cusat_fec_ctx*> codes;
-    std::vector<int> variant;  // stream decoder variant of each code
+    std::vector<int> variant;        // stream decoder variant of each code
+    std::vector<int> variant_split;  // its split-edge variant, or -1
     int max_frames = 0;
     unsigned flags = 0;
-    int n_cu = 0, max_wg_per_cu = 1, lds = 0;
+    int n_cu = 0;
+    int max_wg[2] = {0, 0};  // workgroups per CU of the 6-wave / 12-wave kernel (0: does not fit)
+    int lds[2] = {0, 0};     // dynamic LDS bytes of each, control ints included
     size_t ws_v4 = 0;  // uint4 per workgroup
     // device
     code_desc* d_codes = nullptr;
@@ -303,6 +349,7 @@ int upload_codes(cusat_stream_ctx* sc)
         o.n_param_iters = c->n_param_iters;
         o.learned_trials = c->n_param_iters;
         o.variant = sc->variant[k];
+        o.variant_split = sc->variant_split[k];
         o.bch_bytes = c->fp.n_bch / 8;
         o.bch_degree = c->bch.degree;
         o.bch_chunk = c->bch_chunk;
@@ -313,6 +360,27 @@ int upload_codes(cusat_stream_ctx* sc)
 }
 
 }  // namespace
+
+// raise the kernel's dynamic-LDS limit to `lds` (the attribute is per
+// function: only ever raised, another stream may need more) and report how
+// many workgroups fit on a CU (0 if none)
+template <bool WIDE>
+hipError_t fit_kernel(int lds, int* occ)
+{
+    static int configured = 0;
+    auto kern = &ldpc_stream_kernel<WIDE>;
+    *occ = 0;
+    if (lds > __atomic_load_n(&configured, __ATOMIC_ACQUIRE)) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return hipSuccess;  // does not fit: occ stays 0
+        }
+        __atomic_store_n(&configured, lds, __ATOMIC_RELEASE);
+    }
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, kern, stream_threads<WIDE>(), lds);
+}
 
 extern "C" int cusat_stream_create(cusat_stream_ctx** out, cusat_fec_ctx* const* codes, int n_codes,
                                    int max_frames, unsigned flags)
@@ -352,10 +420,21 @@ extern "C" int cusat_stream_create(cusat_stream_ctx** out, cusat_fec_ctx* const*
         }
         sc->codes.push_back(c);
         sc->variant.push_back(variant_id(cmax, exact));
+        const int vs = split_variant_id(cmax, exact);
+        sc->variant_split.push_back(vs);
         rec_max = std::max(rec_max, ldpc_msg_bytes(c->geo.q, cmax));
-        const int lds = ldpc_lds_bytes(c->geo.K, c->geo.R, c->geo.n_slots, c->kern_steps, (int)c->geo.groups.size());
-        sc->lds = std::max(sc->lds, std::max(lds, 256 * 4 * 8));
+        const int ng = (int)c->geo.groups.size();
+        const int lds = ldpc_lds_bytes(c->geo.K, c->geo.R, c->geo.n_slots, c->kern_steps, ng);
+        int lds_w = lds;
+        if (vs >= 0) {
+            rec_max = std::max(rec_max, ldpc_msg_bytes_split(c->geo.q, cmax));
+            lds_w = ldpc_lds_bytes_split(c->geo.K, c->geo.R, c->geo.n_slots, c->kern_steps, ng, 2);
+        }
+        // + the BCH byte table (up to 256 x 4 words) staged in the LDS
+        sc->lds[0] = std::max(sc->lds[0], std::max(lds, 256 * 4 * 8));
+        sc->lds[1] = std::max(sc->lds[1], std::max(lds_w, 256 * 4 * 8));
     }
+    for (int w = 0; w < 2; w++) sc->lds[w] += 16;  // control ints
     int dev = 0;
     hipGetDevice(&dev);
     hipDeviceProp_t prop;
@@ -365,26 +444,15 @@ extern "C" int cusat_stream_create(cusat_stream_ctx** out, cusat_fec_ctx* const*
         return CUSAT_E_HIP;
     }
     sc->n_cu = prop.multiProcessorCount;
-    // the attribute is per function: only ever raise it (another context may
-    // need more than this one)
-    static int configured = 0;
-    hipError_t e = hipSuccess;
-    if (sc->lds > __atomic_load_n(&configured, __ATOMIC_ACQUIRE)) {
-        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&ldpc_stream_kernel),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, sc->lds);
-        if (e == hipSuccess) __atomic_store_n(&configured, sc->lds, __ATOMIC_RELEASE);
-    }
-    int occ = 0;
-    if (e == hipSuccess)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, ldpc_stream_kernel, STREAM_THREADS, sc->lds);
-    if (e != hipSuccess || occ < 1) {
+    hipError_t e = fit_kernel<false>(sc->lds[0], &sc->max_wg[0]);
+    if (e == hipSuccess) e = fit_kernel<true>(sc->lds[1], &sc->max_wg[1]);
+    if (e != hipSuccess || sc->max_wg[0] < 1) {
         delete sc;
-        set_last_error("stream kernel does not fit (%d bytes LDS): %s", sc->lds, hipGetErrorString(e));
+        set_last_error("stream kernel does not fit (%d bytes LDS): %s", sc->lds[0], hipGetErrorString(e));
         return CUSAT_E_HIP;
     }
-    sc->max_wg_per_cu = occ;
     sc->ws_v4 = rec_max / 16;
-    const size_t n_wg = (size_t)sc->n_cu * occ;
+    const size_t n_wg = (size_t)sc->n_cu * std::max(sc->max_wg[0], sc->max_wg[1]);
     size_t bytes = sizeof(code_desc) * n_codes + sizeof(cusat_stream_frame) * max_frames +
                    sizeof(int2) * max_frames + sizeof(int) * max_frames + 256 + 4096;
     char* base = nullptr;
@@ -411,7 +479,11 @@ extern "C" int cusat_stream_create(cusat_stream_ctx** out, cusat_fec_ctx* const*
         set_last_error("counter init failed");
         return CUSAT_E_HIP;
     }
-    sc->tuning.wg_per_cu = occ;
+    // the 12-wave kernel needs a split variant of every code
+    if (std::any_of(sc->variant_split.begin(), sc->variant_split.end(), [](int v) { return v < 0; }))
+        sc->max_wg[1] = 0;
+    sc->tuning.split_edges = sc->max_wg[1] >= 1 ? 1 : 0;
+    sc->tuning.wg_per_cu = sc->max_wg[sc->tuning.split_edges];
     sc->tuning.frames_per_job = 1;
     sc->tuning.order = CUSAT_STREAM_ORDER_COST;
     *out = sc;
@@ -435,9 +507,11 @@ extern "C" int cusat_stream_plan(cusat_stream_ctx* sc, const cusat_stream_frame*
         return CUSAT_E_INVALID;
     }
     cusat_stream_tuning tu = tuning ? *tuning : sc->tuning;
-    if (tu.wg_per_cu < 1 || tu.wg_per_cu > sc->max_wg_per_cu || tu.frames_per_job < 1 ||
-        tu.frames_per_job > 64 || (tu.order != CUSAT_STREAM_ORDER_GIVEN && tu.order != CUSAT_STREAM_ORDER_COST)) {
-        set_last_error("tuning out of range (wg_per_cu 1..%d, frames_per_job 1..64)", sc->max_wg_per_cu);
+    const int wmax = (tu.split_edges == 0 || tu.split_edges == 1) ? sc->max_wg[tu.split_edges] : 0;
+    if (tu.wg_per_cu < 1 || tu.wg_per_cu > wmax || tu.frames_per_job < 1 || tu.frames_per_job > 64 ||
+        (tu.order != CUSAT_STREAM_ORDER_GIVEN && tu.order != CUSAT_STREAM_ORDER_COST)) {
+        set_last_error("tuning out of range (split_edges 0 / 1, wg_per_cu 1..%d / 1..%d, frames_per_job 1..64)",
+                       sc->max_wg[0], sc->max_wg[1]);
         return CUSAT_E_INVALID;
     }
     for (int k = 0; k < n_frames; k++)
@@ -459,7 +533,7 @@ extern "C" int cusat_stream_get_tuning(const cusat_stream_ctx* sc, cusat_stream_
 {
     if (!sc || !out) { set_last_error("bad argument"); return CUSAT_E_INVALID; }
     *out = sc->tuning;
-    if (max_wg_per_cu) *max_wg_per_cu = sc->max_wg_per_cu;
+    if (max_wg_per_cu) *max_wg_per_cu = sc->max_wg[sc->tuning.split_edges];
     return CUSAT_OK;
 }
 
@@ -469,10 +543,18 @@ extern "C" int cusat_stream_decode(cusat_stream_ctx* sc, const int16_t* llr, uin
     if (!sc || !llr || !hard || !trials || !valid) { set_last_error("bad argument"); return CUSAT_E_INVALID; }
     if (sc->n_jobs == 0) return CUSAT_OK;
     const int grid = sc->n_cu * sc->tuning.wg_per_cu;
-    hipLaunchKernelGGL(ldpc_stream_kernel, dim3(grid), dim3(STREAM_THREADS), sc->lds, (hipStream_t)stream,
-                       sc->d_codes, sc->d_frames, sc->d_jobs, sc->d_order, sc->n_jobs, sc->d_counters, llr, hard,
-                       trials, valid, fallback, sc->d_ws, sc->ws_v4,
-                       (sc->flags & CUSAT_STREAM_LEARNED) ? 1 : 0);
+    const int w = sc->tuning.split_edges;
+    const int learned = (sc->flags & CUSAT_STREAM_LEARNED) ? 1 : 0;
+    if (w)
+        hipLaunchKernelGGL(ldpc_stream_kernel<true>, dim3(grid), dim3(stream_threads<true>()), sc->lds[1],
+                           (hipStream_t)stream, sc->d_codes, sc->d_frames, sc->d_jobs, sc->d_order, sc->n_jobs,
+                           sc->d_counters, llr, hard, trials, valid, fallback, sc->d_ws, sc->ws_v4, learned,
+                           sc->lds[1] - 16);
+    else
+        hipLaunchKernelGGL(ldpc_stream_kernel<false>, dim3(grid), dim3(stream_threads<false>()), sc->lds[0],
+                           (hipStream_t)stream, sc->d_codes, sc->d_frames, sc->d_jobs, sc->d_order, sc->n_jobs,
+                           sc->d_counters, llr, hard, trials, valid, fallback, sc->d_ws, sc->ws_v4, learned,
+                           sc->lds[0] - 16);
     HIP_TRY(hipGetLastError());
     return CUSAT_OK;
 }
@@ -489,10 +571,11 @@ extern "C" int cusat_stream_autotune(cusat_stream_ctx* sc, const int16_t* llr, u
     cusat_stream_tuning best = sc->tuning;
     float best_t = 1e30f;
     int rc = CUSAT_OK;
+    for (int split = 0; split < 2 && rc == CUSAT_OK; split++)
     for (int order = 0; order < 2 && rc == CUSAT_OK; order++)
         for (int fpj : {1, 2, 4})
-            for (int wg = 1; wg <= sc->max_wg_per_cu && rc == CUSAT_OK; wg++) {
-                cusat_stream_tuning tu{wg, fpj, order, 0};
+            for (int wg = 1; wg <= sc->max_wg[split] && rc == CUSAT_OK; wg++) {
+                cusat_stream_tuning tu{wg, fpj, order, split};
                 rc = build_plan(sc, tu);
                 if (rc != CUSAT_OK) break;
                 rc = cusat_stream_decode(sc, llr, hard, trials, valid, fallback, stream);  // warm

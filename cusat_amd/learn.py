@@ -249,6 +249,88 @@ def evaluate(params_path=DEFAULT_PARAMS, points=(8.8, 9.0, 9.2, 9.4, 9.6), frame
     return rows
 
 
+def evaluate_stream(params_path=DEFAULT_PARAMS, points=(9.0, 9.2, 9.4), frames=32768, learned_trials=20,
+                    plain_trials=50, pool_frames=2048, seed=777, reps=3, log=print):
+    """The plain decoder vs the learned decoder with plain fallback as a
+    receiver runs them on a continuous stream: all `frames` frames of a
+    point (LLRs resident in HBM) decoded by ONE persistent launch
+    (cusat_stream_decode: decode, BCH and, in learned mode, the plain
+    re-decode of failing frames inside the same workgroup), timed with HIP
+    events as the mean of `reps` launches.  Frame-by-frame results are
+    compared with the batched decoders (cusat_ldpc_decode + cusat_bch_check,
+    cusat_rx_fec_learned) on the same LLRs."""
+    import torch
+    import cusat_amd as ca
+    d, bq, aq = load_params(params_path)
+    fs, rate, mod = d["framesize"], d["rate"], d["modulation"]
+    chunk = min(1024, frames)
+    ctx = ca.FecContext(fs, rate, mod, max_trials=plain_trials, max_batch=frames)
+    ctx.set_params(bq, aq)
+    inf = ctx.info
+    N, K, kb = inf["n_ldpc"], inf["k_ldpc"], inf["k_bch"]
+    pool = FramePool(ctx, fs, rate, mod, pool_frames, seed)
+    dev = torch.device("cuda")
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    rng = np.random.default_rng(seed)
+    llr = torch.empty((frames, N), dtype=torch.int16, device=dev)
+    hard = torch.empty((frames, K // 8), dtype=torch.uint8, device=dev)
+    tr = torch.empty(frames, dtype=torch.int32, device=dev)
+    va = torch.empty(frames, dtype=torch.int32, device=dev)
+    fb = torch.empty(frames, dtype=torch.int32, device=dev)
+    plan = np.zeros(frames, ca.STREAM_FRAME)
+    plan["llr_offset"] = np.arange(frames, dtype=np.int64) * N
+    plan["hard_offset"] = np.arange(frames, dtype=np.int64) * (K // 8)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    streams = {m: ca.StreamContext([ctx], frames, f)
+               for m, f in (("plain", 0), ("learned+fallback", ca.STREAM_LEARNED))}
+    for sc in streams.values():
+        sc.plan(plan)
+    rows = []
+    for es in points:
+        idx = torch.from_numpy(rng.integers(0, pool.n, frames)).to(dev)
+        for c0 in range(0, frames, chunk):
+            pool.llr(idx[c0:c0 + chunk], np.full((chunk + 31) // 32, es), gen, llr[c0:c0 + chunk])
+        info_ref = pool.info[idx]
+        for m, sc in streams.items():
+            sc.decode(llr, hard, tr, va, fb)
+            torch.cuda.synchronize()
+            ev[0].record()
+            for _ in range(reps):
+                sc.decode(llr, hard, tr, va, fb)
+            ev[1].record()
+            torch.cuda.synchronize()
+            ms = ev[0].elapsed_time(ev[1]) / reps
+            # the batched decoders on the same LLRs, chunk by chunk
+            same = True
+            h2 = torch.empty((chunk, K // 8), dtype=torch.uint8, device=dev)
+            t2, v2, f2 = [torch.empty(chunk, dtype=torch.int32, device=dev) for _ in range(3)]
+            for c0 in range(0, frames, chunk):
+                sl = slice(c0, c0 + chunk)
+                if m == "plain":
+                    ctx.decode(llr[sl], h2, t2, chunk)
+                    ctx.bch_check(h2, v2, chunk)
+                else:
+                    ctx.rx_fec_learned(llr[sl], h2, t2, v2, f2, chunk, learned_trials)
+                    same &= bool(torch.equal(f2, fb[sl]))
+                same &= bool(torch.equal(h2, hard[sl]) and torch.equal(t2, tr[sl]) and torch.equal(v2, va[sl]))
+            ok = (tr >= 0) & (va == 1) & (hard[:, : kb // 8] == info_ref).all(1)
+            it = torch.where(tr >= 0, plain_trials - tr, torch.full_like(tr, plain_trials))
+            if m != "plain":
+                it = torch.where(fb == 1, it + learned_trials, learned_trials - tr)
+            row = {"es_n0_db": es, "decoder": m + " (stream)", "frames": frames,
+                   "fer": float((~ok).float().mean()), "mean_iterations": float(it.float().mean()),
+                   "gbps_info": frames * kb / (ms * 1e-3) / 1e9, "ms_launch": round(ms, 3),
+                   "identical_to_batched": same}
+            if m != "plain":
+                row["fallback_rate"] = float(fb.float().mean())
+            rows.append(row)
+            log(json.dumps(row))
+    for sc in streams.values():
+        sc.close()
+    return rows
+
+
 def train_set(codes="s2x", steps=200, batch=512, pool_frames=1024, out_dir=LEARNED_DIR, margin=0.3,
               eval_frames=2048, force=False, only=None, log=print):
     """Train one parameter file per (frame size, MODCOD) of a mixed stream
@@ -333,6 +415,8 @@ def main(argv=None):
     e.add_argument("--frames", type=int, default=16384)
     e.add_argument("--points", default="8.8,9.0,9.2,9.4,9.6")
     e.add_argument("--out", default=None)
+    e.add_argument("--stream", action="store_true",
+                   help="one persistent launch per point over all frames (plain vs learned + fallback)")
     a = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)
     if a.cmd == "train":
@@ -347,9 +431,12 @@ def main(argv=None):
         train_set(a.codes, a.steps, a.batch, a.pool, a.out_dir, eval_frames=a.eval_frames, force=a.force,
                   only=a.only.split(",") if a.only else None, log=log)
     else:
-        rows = evaluate(a.params, [float(x) for x in a.points.split(",")], a.frames, log=log)
-        summary = {m: fer_crossing(rows, m)
-                   for m in ("plain", "learned", "learned+fallback", "identity+fallback")}
+        pts = [float(x) for x in a.points.split(",")]
+        if a.stream:
+            rows = evaluate_stream(a.params, pts, a.frames, log=log)
+        else:
+            rows = evaluate(a.params, pts, a.frames, log=log)
+        summary = {m: fer_crossing(rows, m) for m in sorted(set(r["decoder"] for r in rows))}
         log(json.dumps({"fer_1e-3_crossing": summary}))
         if a.out:
             with open(a.out, "w") as f:

@@ -224,7 +224,8 @@ typedef struct cusat_stream_tuning {
     int wg_per_cu;         /* persistent workgroups per CU (1..max the LDS allows) */
     int frames_per_job;    /* consecutive same-code frames per queue fetch */
     int order;             /* CUSAT_STREAM_ORDER_* */
-    int reserved;
+    int split_edges;       /* 1: 12-wave workgroups, a check's edges split over two
+                              wave groups for codes that have that variant; 0: 6-wave */
 } cusat_stream_tuning;
 /* stream flag: codes whose context has learned parameters
  * (cusat_ldpc_set_params, n_iters > 0) decode n_iters learned iterations,
@@ -245,7 +246,7 @@ int cusat_stream_get_tuning(const cusat_stream_ctx* sc, cusat_stream_tuning* out
  * frame number. */
 int cusat_stream_decode(cusat_stream_ctx* sc, const int16_t* llr, uint8_t* hard, int32_t* trials,
                         int32_t* valid, int32_t* fallback, void* stream);
-/* Time every tuning candidate (workgroups per CU, frames per job, order)
+/* Time every tuning candidate (split_edges, workgroups per CU, frames per job, order)
  * over reps launches on the given buffers, keep the fastest (results do not
  * depend on the tuning) and report it with its time per launch. */
 int cusat_stream_autotune(cusat_stream_ctx* sc, const int16_t* llr, uint8_t* hard, int32_t* trials,

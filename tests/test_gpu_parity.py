@@ -65,6 +65,17 @@ def test_short_frames(rate, mod, es):
     _chain_parity(1, rate, mod, es, 32, 9)
 
 
+@pytest.mark.parametrize("fs,rate,mod,es", [(0, "135/180", "128APSK", 18.2), (0, "154/180", "QPSK", 6.4),
+                                             (0, "96/180", "8PSK", 5.0), (0, "22/30", "16APSK", 10.6),
+                                             (0, "7/9", "32APSK", 12.9), (1, "7/15", "QPSK", 2.0)])
+def test_s2x_codes_with_uneven_layers(fs, rate, mod, es):
+    """DVB-S2X codes whose layers hold different numbers of data circulants
+    (the guarded decoder variants, CMAX 12 / 18 / 30, two-lane and one-lane
+    schedules), on the split-edge kernel and on the 6-wave kernel."""
+    _chain_parity(fs, rate, mod, es, 12, 21)
+    _chain_parity(fs, rate, mod, es, 12, 21, flags=ca.CTX_NO_SPLIT_KERNEL)
+
+
 def test_sequential_schedule_flag():
     """CUSAT_CTX_SEQUENTIAL_SCHEDULE = the reference's CUSAT_LDPC_SEQ=1."""
     _chain_parity(0, "3/4", "8PSK", 8.0, 16, 3, flags=ca.CTX_SEQUENTIAL_SCHEDULE)

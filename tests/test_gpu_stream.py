@@ -4,7 +4,7 @@ A stream of frames of several MODCODs (normal and short FECFRAMEs,
 interleaved, LLRs packed back to back) decoded in one launch must give, frame
 for frame, exactly what each frame's own context gives (cusat_ldpc_decode +
 cusat_bch_check; cusat_rx_fec_learned for learned codes), whatever the
-tuning; a hipGraph capture of the launch replays correctly with new LLRs.
+tuning (6-wave or 12-wave split-edge workgroups included); a hipGraph capture of the launch replays correctly with new LLRs.
 """
 import numpy as np
 import pytest
@@ -98,8 +98,10 @@ def check(ctxs, frames, who, ref, got):
         np.testing.assert_array_equal(hard[o:o + kb], h0[j])
 
 
-@pytest.mark.parametrize("tuning", [None, {"wg_per_cu": 1, "frames_per_job": 4, "order": 0},
-                                    {"wg_per_cu": 1, "frames_per_job": 2, "order": 1}])
+@pytest.mark.parametrize("tuning", [None, {"wg_per_cu": 1, "frames_per_job": 4, "order": 0, "split_edges": 0},
+                                    {"wg_per_cu": 1, "frames_per_job": 2, "order": 1, "split_edges": 0},
+                                    {"wg_per_cu": 1, "frames_per_job": 1, "order": 1, "split_edges": 1},
+                                    {"wg_per_cu": 1, "frames_per_job": 3, "order": 0, "split_edges": 1}])
 def test_stream_matches_per_code_decoders(tuning):
     ctxs, frames, llr, by_code, who, hb = build_stream(MIX, 5, 3)
     ref = per_code_reference(ctxs, by_code)
@@ -139,8 +141,9 @@ def test_stream_learned_matches_rx_fec_learned():
     ref = per_code_reference(ctxs, by_code, learned_trials=bq.shape[0])
     assert ref[0][3].any() and not ref[0][3].all(), "the case needs frames with and without fallback"
     sc = ca.StreamContext(ctxs, len(frames), ca.STREAM_LEARNED)
-    sc.plan(frames)
-    check(ctxs, frames, who, ref, run_stream(sc, frames, llr, hb))
+    for split in (0, 1):
+        sc.plan(frames, {"wg_per_cu": 1, "frames_per_job": 1, "order": 1, "split_edges": split})
+        check(ctxs, frames, who, ref, run_stream(sc, frames, llr, hb))
 
 
 def test_stream_graph_replays():
