@@ -159,6 +159,10 @@ __device__ __forceinline__ int wg_any(int v, int* flags, int& k)
 // VN group's first pair dword, number of slots, the slots' dword indices.
 constexpr int GROUP_WORDS = 8;
 
+// word 3 of a raw buffer resource: 32-bit data format, no swizzle / stride;
+// loads beyond num_records bytes return 0
+constexpr int BUF_RSRC_WORD3 = 0x00020000;
+
 // Decode one frame with the whole workgroup (LANES x 192 threads; a
 // workgroup of 384 threads runs a LANES = 1 code with its second lane idle,
 // since every step of a one-lane schedule has lane 1 = -1).  llr: the frame's
@@ -227,7 +231,23 @@ __device__ __forceinline__ int decode_frame(unsigned char* lds_raw, const int16_
     // next step into rb, odd steps the reverse (the host pads the schedule
     // to an even step count).  The prefetch is unconditional, so the only
     // vector-memory wait in a step is for the record loaded one step earlier.
+    // learned parameters, loaded one step ahead with the records (vector
+    // buffer loads; iterations >= n_param_iters are plain and load nothing)
+    const int n_par = p.n_param_iters * q;
+    const __amdgpu_buffer_rsrc_t rs_beta =
+        __builtin_amdgcn_make_buffer_rsrc((void*)beta, (short)0, beta ? n_par * 4 : 0, BUF_RSRC_WORD3);
+    const __amdgpu_buffer_rsrc_t rs_alpha =
+        __builtin_amdgcn_make_buffer_rsrc((void*)alpha, (short)0, alpha ? n_par * 4 : 0, BUF_RSRC_WORD3);
+    auto par_load = [&](int itx, int ix, int& b, int& a) {
+        b = 1;
+        a = 0;
+        if (itx < p.n_param_iters) {
+            if (beta) b = __builtin_amdgcn_raw_buffer_load_b32(rs_beta, (itx * q + ix) * 4, 0, 0);
+            if (alpha) a = __builtin_amdgcn_raw_buffer_load_b32(rs_alpha, (itx * q + ix) * 4, 0, 0);
+        }
+    };
     uint4 ra[R::V4], rb[R::V4];
+    int pb[2], pa[2];  // beta, alpha of the step whose record is in ra / rb
     {
         const int i0 = max(__builtin_amdgcn_readfirstlane(sched_l[lane]), 0);
         const uint4* src = rec_base + ((size_t)i0 * HALF + tl) * R::V4;
@@ -236,6 +256,9 @@ __device__ __forceinline__ int decode_frame(unsigned char* lds_raw, const int16_
             ra[k] = src[k];
             rb[k] = make_uint4(0, 0, 0, 0);
         }
+        par_load(0, i0, pb[0], pa[0]);
+        pb[1] = 1;
+        pa[1] = 0;
     }
 
     int res = -1;
@@ -293,11 +316,11 @@ __device__ __forceinline__ int decode_frame(unsigned char* lds_raw, const int16_
                 const uint4* src = rec_base + ((size_t)inx * HALF + tl) * R::V4;
 #pragma unroll
                 for (int k = 0; k < R::V4; k++) nxt[k] = src[k];
+                par_load(s + 1 < p.n_steps ? it : it + 1, inx, pb[hs ^ 1], pa[hs ^ 1]);
             }
             if (act && i >= 0) {
-                int bt = 1, al = 0;
-                if (beta && it < p.n_param_iters) bt = beta[it * q + i];
-                if (alpha && it < p.n_param_iters) al = alpha[it * q + i];
+                const int bt = __builtin_amdgcn_readfirstlane(pb[hs]);
+                const int al = __builtin_amdgcn_readfirstlane(pa[hs]);
                 const u2 beta2 = as_u2(splat(bt));
                 const int cnt = EXACT ? CMAX : lay_cnt[i];
                 const circ_rec* cc = circ + lay_off[i];

@@ -100,7 +100,25 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
             for (int k = 0; k < R::V4; k++) dst[k] = make_uint4(0, 0, 0, 0);
         }
     }
+    // Learned parameters of a (iteration, layer): vector buffer loads issued
+    // one step ahead with the message-record prefetch (a scalar load after
+    // the step's barrier would stall every wave of the workgroup); loads
+    // past the table (iterations >= n_param_iters are plain) are not made.
+    const int n_par = p.n_param_iters * q;
+    const __amdgpu_buffer_rsrc_t rs_beta =
+        __builtin_amdgcn_make_buffer_rsrc((void*)beta, (short)0, beta ? n_par * 4 : 0, BUF_RSRC_WORD3);
+    const __amdgpu_buffer_rsrc_t rs_alpha =
+        __builtin_amdgcn_make_buffer_rsrc((void*)alpha, (short)0, alpha ? n_par * 4 : 0, BUF_RSRC_WORD3);
+    auto par_load = [&](int itx, int ix, int& b, int& a) {
+        b = 1;
+        a = 0;
+        if (itx < p.n_param_iters) {
+            if (beta) b = __builtin_amdgcn_raw_buffer_load_b32(rs_beta, (itx * q + ix) * 4, 0, 0);
+            if (alpha) a = __builtin_amdgcn_raw_buffer_load_b32(rs_alpha, (itx * q + ix) * 4, 0, 0);
+        }
+    };
     uint4 ra[R::V4], rb[R::V4];
+    int pb[2], pa[2];  // beta, alpha of the step whose record is in ra / rb
     {
         const int i0 = max(__builtin_amdgcn_readfirstlane(sched_l[lane]), 0);
         const uint4* src = rec_at(i0, tl);
@@ -109,6 +127,9 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
             ra[k] = src[k];
             rb[k] = make_uint4(0, 0, 0, 0);
         }
+        par_load(0, i0, pb[0], pa[0]);
+        pb[1] = 1;
+        pa[1] = 0;
     }
 
     int res = -1;
@@ -167,6 +188,7 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                 const uint4* src = rec_at(inx, tl);
 #pragma unroll
                 for (int k = 0; k < R::V4; k++) nxt[k] = src[k];
+                par_load(s + 1 < p.n_steps ? it : it + 1, inx, pb[hs ^ 1], pa[hs ^ 1]);
             }
             const bool work = act && i >= 0;
             const s2 zero = splat(0);
@@ -239,9 +261,8 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                     mn1 = __builtin_elementwise_min(mn1, om1);
                     sg ^= xsg[po];
                 }
-                int bt = 1, al = 0;
-                if (beta && it < p.n_param_iters) bt = beta[it * q + i];
-                if (alpha && it < p.n_param_iters) al = alpha[it * q + i];
+                const int bt = __builtin_amdgcn_readfirstlane(pb[hs]);
+                const int al = __builtin_amdgcn_readfirstlane(pa[hs]);
                 const u2 beta2 = as_u2(splat(bt));
                 // offset once per check: max(min|x| - beta, 0)
                 const s2 b1 = as_s2u(__builtin_elementwise_sub_sat(as_u2(mn1), beta2));
