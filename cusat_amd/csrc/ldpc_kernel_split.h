@@ -55,8 +55,10 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
     uint32_t* par32 = app32 + p.K / 2;                        // R/2 dwords
     uint32_t* dsl32 = par32 + p.R / 2;                        // n_slots * 180 dwords
     uint32_t* xch = dsl32 + p.n_slots * HALF;                 // [3][LANES][2][180] partials
-    int* sched_l = reinterpret_cast<int*>(xch + 3 * LANES * 2 * HALF);
-    int* goff_l = sched_l + 2 * p.n_steps;
+    // per (step, lane): layer (-1 idle), its lay_off and data-circulant count,
+    // so that a step's first global access is its circulant records
+    int4* sched_l = reinterpret_cast<int4*>(xch + 3 * LANES * 2 * HALF);
+    int* goff_l = reinterpret_cast<int*>(sched_l + 2 * p.n_steps);
     int* grp_l = goff_l + p.n_steps + 1;
     int* flg = grp_l + GROUP_WORDS * p.n_groups;  // [3] wg_any flags
     int16_t* app16 = reinterpret_cast<int16_t*>(lds_raw);
@@ -77,7 +79,11 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
     uint32_t* xm2 = xch + LANES * 2 * HALF;
     uint32_t* xsg = xch + 2 * LANES * 2 * HALF;
 
-    for (int k = t; k < 2 * p.n_steps; k += NTHR) sched_l[k] = steps[k];
+    for (int k = t; k < 2 * p.n_steps; k += NTHR) {
+        const int i = steps[k];
+        const int ic = max(i, 0);
+        sched_l[k] = make_int4(i, lay_off[ic], EXACT ? CMAX : lay_cnt[ic], 0);
+    }
     for (int k = t; k <= p.n_steps; k += NTHR) goff_l[k] = grp_off[k];
     for (int k = t; k < GROUP_WORDS * p.n_groups; k += NTHR) grp_l[k] = groups[k];
     if (t < 3) flg[t] = 0;
@@ -93,7 +99,7 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
     const int tl = act ? tt : HALF - 1;
     auto rec_at = [&](int i, int pair) { return rec_base + (((size_t)i * 2 + half) * HALF + pair) * R::V4; };
     for (int s = 0; s < p.n_steps; s++) {
-        const int i = __builtin_amdgcn_readfirstlane(sched_l[2 * s + lane]);
+        const int i = __builtin_amdgcn_readfirstlane(sched_l[2 * s + lane].x);
         if (act && i >= 0) {
             uint4* dst = rec_at(i, tt);
 #pragma unroll
@@ -119,8 +125,16 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
     };
     uint4 ra[R::V4], rb[R::V4];
     int pb[2], pa[2];  // beta, alpha of the step whose record is in ra / rb
+    int si[2], so[2], sc[2];  // its layer, lay_off and circulant count
     {
-        const int i0 = max(__builtin_amdgcn_readfirstlane(sched_l[lane]), 0);
+        const int4 e0 = sched_l[lane];
+        si[0] = __builtin_amdgcn_readfirstlane(e0.x);
+        so[0] = __builtin_amdgcn_readfirstlane(e0.y);
+        sc[0] = __builtin_amdgcn_readfirstlane(e0.z);
+        si[1] = -1;
+        so[1] = 0;
+        sc[1] = 0;
+        const int i0 = max(si[0], 0);
         const uint4* src = rec_at(i0, tl);
 #pragma unroll
         for (int k = 0; k < R::V4; k++) {
@@ -137,7 +151,8 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
         // ---- syndrome (half 0 evaluates whole check pairs), early exit ----
         int ok = 1;
         for (int s = 0; s < p.n_steps; s++) {
-            const int i = __builtin_amdgcn_readfirstlane(sched_l[2 * s + lane]);
+            const int4 se = sched_l[2 * s + lane];
+            const int i = __builtin_amdgcn_readfirstlane(se.x);
             int bad = 0;
             if (act && i >= 0 && half == 0) {
                 uint32_t sg = par32[i * HALF + tt];
@@ -147,8 +162,8 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                     const uint32_t w = par32[(q - 1) * HALF + (tt > 0 ? tt - 1 : HALF - 1)];
                     sg ^= tt > 0 ? w : (w << 16);
                 }
-                const circ_rec* cc = circ + lay_off[i];
-                const int cnt = EXACT ? CMAX : lay_cnt[i];
+                const circ_rec* cc = circ + __builtin_amdgcn_readfirstlane(se.y);
+                const int cnt = EXACT ? CMAX : __builtin_amdgcn_readfirstlane(se.z);
 #pragma unroll
                 for (int e = 0; e < CMAX; e++) {
                     if (EXACT || e < cnt) {
@@ -173,7 +188,10 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
             const int s = s0 + hs;
             uint4* cur = hs == 0 ? ra : rb;
             uint4* nxt = hs == 0 ? rb : ra;
-            const int i = __builtin_amdgcn_readfirstlane(sched_l[2 * s + lane]);
+            // this step's schedule entry, read one step earlier
+            const int i = si[hs];
+            const int loff = so[hs];
+            const int lcnt = sc[hs];
             uint32_t rw[R::WORDS];
 #pragma unroll
             for (int k = 0; k < R::V4; k++) {
@@ -184,7 +202,11 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
             }
             {
                 const int sn = s + 1 < p.n_steps ? s + 1 : 0;
-                const int inx = max(__builtin_amdgcn_readfirstlane(sched_l[2 * sn + lane]), 0);
+                const int4 en = sched_l[2 * sn + lane];
+                si[hs ^ 1] = __builtin_amdgcn_readfirstlane(en.x);
+                so[hs ^ 1] = __builtin_amdgcn_readfirstlane(en.y);
+                sc[hs ^ 1] = __builtin_amdgcn_readfirstlane(en.z);
+                const int inx = max(si[hs ^ 1], 0);
                 const uint4* src = rec_at(inx, tl);
 #pragma unroll
                 for (int k = 0; k < R::V4; k++) nxt[k] = src[k];
@@ -193,7 +215,7 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
             const bool work = act && i >= 0;
             const s2 zero = splat(0);
             // data edges of the layer and of this half (wave-uniform)
-            const int cnt = EXACT ? CMAX : __builtin_amdgcn_readfirstlane(lay_cnt[max(i, 0)]);
+            const int cnt = EXACT ? CMAX : lcnt;
             const int n0 = EXACT ? NE0 : (cnt + 1) >> 1;
             const int ne = half ? cnt - n0 : n0;
             s2 xv[NE0], mv[NE0], av[NE0], ov[NE0];
@@ -210,7 +232,7 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                                        : (i > 0 ? (i - 1) * HALF + tt : (q - 1) * HALF + (tt > 0 ? tt - 1 : HALF - 1));
             const bool dummy = half == 1 && i == 0 && tt == 0;
             s2 xp = zero, mp = zero;
-            const circ_rec* cc = circ + lay_off[max(i, 0)] + half * n0;
+            const circ_rec* cc = circ + loff + half * n0;
             if (work) {
 #pragma unroll
                 for (int e = 0; e < NE0; e++) {
