@@ -31,20 +31,27 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
+# generated header -> (symbol, data file) pairs it embeds
+DATA_HEADERS = {
+    "data_inc.h": (("k_ldpc_tables_txt", "ldpc_tables.txt"), ("k_ldpc_schedules_txt", "ldpc_schedules.txt")),
+    "apsk256_inc.h": (("k_apsk256_points_txt", "apsk256_points.txt"),),
+}
+
+
 def gen_data_header():
     out_dir = os.path.join(CSRC, "_gen")
     os.makedirs(out_dir, exist_ok=True)
-    parts = []
-    for sym, fname in (("k_ldpc_tables_txt", "ldpc_tables.txt"),
-                       ("k_ldpc_schedules_txt", "ldpc_schedules.txt")):
-        data = open(os.path.join(PKG, "data", fname), "rb").read()
-        body = ",".join(str(b) for b in data)
-        parts.append("static const char %s[] = {%s,0};\n" % (sym, body))
-    path = os.path.join(out_dir, "data_inc.h")
-    text = "// generated by cusat_amd/build.py from cusat_amd/data/*.txt\n#pragma once\n" + "".join(parts)
-    if not os.path.exists(path) or open(path).read() != text:
-        with open(path, "w") as f:
-            f.write(text)
+    for header, items in DATA_HEADERS.items():
+        parts = []
+        for sym, fname in items:
+            data = open(os.path.join(PKG, "data", fname), "rb").read()
+            body = ",".join(str(b) for b in data)
+            parts.append("static const char %s[] = {%s,0};\n" % (sym, body))
+        path = os.path.join(out_dir, header)
+        text = "// generated by cusat_amd/build.py from cusat_amd/data/*.txt\n#pragma once\n" + "".join(parts)
+        if not os.path.exists(path) or open(path).read() != text:
+            with open(path, "w") as f:
+                f.write(text)
 
 
 def compile_one(src):
@@ -66,7 +73,7 @@ def compile_one(src):
 
 
 HOST = os.path.join(PKG, "host")
-HOST_SOURCES = ["ldpc_decode.cc", "rx_fec_chain.cc"]
+HOST_SOURCES = ["ldpc_decode.cc", "rx_fec_chain.cc", "rx_fec_pool.cc"]
 HOST_LIB = os.path.join(PKG, "libcusat_dsp.so")
 CPP_TESTS = os.path.join(ROOT, "tests", "cpp")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
@@ -79,7 +86,7 @@ def _run(cmd, what):
 
 
 def build_host(verbose=True):
-    """The C++ cusat_dsp API (ldpc_decode, rx_fec_chain) over the C ABI:
+    """The C++ cusat_dsp API (ldpc_decode, rx_fec_chain, rx_fec_pool) over the C ABI:
     libcusat_dsp.so next to libcusat_fec.so, and the C++ test programs of
     tests/cpp (tests/cpp/_build/<name>)."""
     flags = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROOT, "include"),

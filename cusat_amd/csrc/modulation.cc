@@ -7,7 +7,11 @@
 #include "modulation.h"
 
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+
+#include "_gen/apsk256_inc.h"  // k_apsk256_points_txt
 
 namespace cusat {
 
@@ -314,12 +318,43 @@ void build_128apsk(constellation* c, int rate)
     }
 }
 
+// The 256 (I, Q) pairs of an explicit-coordinate 256APSK table
+// (cusat_amd/data/apsk256_points.txt, EN 302 307-2 Table 15d), label order.
+bool apsk256_table(const char* code_id, float (*xy)[2])
+{
+    char tag[32];
+    std::snprintf(tag, sizeof tag, "@points %s\n", code_id);
+    const char* p = std::strstr(k_apsk256_points_txt, tag);
+    if (!p) return false;
+    p += std::strlen(tag);
+    for (int i = 0; i < 256; i++) {
+        char* end = nullptr;
+        xy[i][0] = std::strtof(p, &end);
+        if (end == p) return false;
+        p = end;
+        xy[i][1] = std::strtof(p, &end);
+        if (end == p) return false;
+        p = end;
+    }
+    return true;
+}
+
 bool build_256apsk(constellation* c, int rate)
 {
+    // Explicit coordinates (Table 15d) for code ids 20/30 and 22/30 (the
+    // latter also for 132/180, as the reference: constellation.cc:400-414),
+    // scaled to unit mean energy (energy summed in double, one float scale).
+    if (rate == R20_30 || rate == R22_30 || rate == R132_180) {
+        float xy[256][2];
+        if (!apsk256_table(rate == R20_30 ? "20_30" : "22_30", xy)) return false;
+        double energy = 0.0;
+        for (int i = 0; i < 256; i++) energy += (double)xy[i][0] * xy[i][0] + (double)xy[i][1] * xy[i][1];
+        const float scale = (float)std::sqrt(256.0 / energy);
+        for (int i = 0; i < 256; i++) set(c, i, xy[i][0] * scale, xy[i][1] * scale);
+        return true;
+    }
     // Ring form (EN 302 307-2 Tables 15a-15c): eight rings of 32 points,
-    // label = XXX q p aaa, angle (2k - 1) pi / 32 mirrored by (p, q).  The
-    // explicit-coordinate rates (20/30, 22/30, 132/180) are not carried.
-    if (rate == R20_30 || rate == R22_30 || rate == R132_180) return false;
+    // label = XXX q p aaa, angle (2k - 1) pi / 32 mirrored by (p, q).
     double g[7] = {1.791, 2.405, 2.980, 3.569, 4.235, 5.078, 6.536};
     if (rate == R128_180 || rate == R135_180) {
         const double common[5] = {1.794, 2.409, 2.986, 3.579, 4.045};

@@ -13,6 +13,10 @@ are therefore extracted rather than retyped:
   decoder runs for normal frames (``lib/cusat_dsp/cuda/ldpc_schedules/*.h``,
   registry ``ldpc_schedule_registry.h:62-99``).  Layered min-sum results depend
   on the layer order, so bit-exact parity needs the same schedules.
+* ``apsk256_points.txt`` -- the explicit-coordinate DVB-S2X 256APSK
+  constellations of EN 302 307-2 Table 15d (code ids 20/30 and 22/30), as
+  printed in the standard (``lib/cusat_dsp/modulation/apsk256_points.h``);
+  the demapper's LLRs depend on every coordinate.
 
 Usage: python extract_reference_data.py /path/to/cusat
 """
@@ -93,6 +97,19 @@ def main(ref):
             for a, b in pairs:
                 f.write("%s %s\n" % (a, b))
     print("schedules:", len(rates))
+
+    text = open(os.path.join(ref, "lib", "cusat_dsp", "modulation", "apsk256_points.h")).read()
+    with open(os.path.join(HERE, "apsk256_points.txt"), "w") as f:
+        f.write("# EN 302 307-2 Table 15d 256APSK coordinates, unnormalised.\n"
+                "# '@points <code id>' then 256 'I Q' lines, line n = label n (MSB first).\n")
+        for cid in ("20_30", "22_30"):
+            body = re.search(r"APSK256_POINTS_%s\[256\]\[2\] = \{(.*?)\n\};" % cid, text, re.S).group(1)
+            pts = re.findall(r"\{\s*(-?\d+\.\d+)f,\s*(-?\d+\.\d+)f\s*\},\s*//\s*([01]{8})", body)
+            assert len(pts) == 256 and [int(lab, 2) for _, _, lab in pts] == list(range(256)), cid
+            f.write("@points %s\n" % cid)
+            for i, q, _ in pts:
+                f.write("%s %s\n" % (i, q))
+    print("apsk256 tables: 2")
 
 
 if __name__ == "__main__":

@@ -1,13 +1,16 @@
 // Host-side checks of the C++ cusat_dsp API (no GPU needed): the acceptance
 // rule (lib/qa_rx_fec_chain.cc:9-17), the decision-directed SNR estimators
-// (modulation/dd_snr.cc semantics) and argument errors raised before any
-// device work.
+// (modulation/dd_snr.cc semantics), argument errors raised before any
+// device work, and the MODCOD table (acm_config.h).
 #include <cmath>
 #include <random>
 #include <stdexcept>
+#include <string>
 
+#include "cusat_dsp/acm_config.h"
 #include "cusat_dsp/ldpc_decode.h"
 #include "cusat_dsp/rx_fec_chain.h"
+#include "cusat_fec.h"
 #include "qa_util.h"
 
 using namespace cusat_dsp;
@@ -67,6 +70,30 @@ int main()
                  thrown = true;
              }
              QA_CHECK(thrown);
+         }},
+        {"modcod_table",
+         [] {
+             const auto& t = get_supported_modcods();
+             QA_CHECK_EQ(t.size(), (size_t)66);
+             const modcod_def_t* m = find_modcod(14);
+             QA_CHECK(m && m->constellation == MOD_8PSK && m->code_rate == RATE_3_4);
+             QA_CHECK(m && std::string(m->name) == "8psk3/4" && m->min_snr_db == 8.11f);
+             QA_CHECK(find_modcod(0) == nullptr);
+             QA_CHECK(find_modcod(67) == nullptr);
+             const modcod_def_t* l = find_modcod_by_name("16apsk2/3l");
+             QA_CHECK(l && l->modcod_idx == 62 && l->code_rate == RATE_20_30);
+             QA_CHECK(find_modcod_by_name("qpsk7/8") == nullptr);
+             // short frames: DVB-S2 MODCODs whose code exists there
+             QA_CHECK(modcod_supports_framesize(*find_modcod(14), FECFRAME_SHORT));
+             QA_CHECK(!modcod_supports_framesize(*find_modcod(11), FECFRAME_SHORT));  // no short 9/10
+             QA_CHECK(!modcod_supports_framesize(*find_modcod(31), FECFRAME_SHORT));  // S2X index
+             QA_CHECK(modcod_supports_framesize(*find_modcod(31), FECFRAME_NORMAL));
+             // every index unique, and every MODCOD a code this library decodes
+             for (size_t a = 0; a < t.size(); a++) {
+                 for (size_t b = a + 1; b < t.size(); b++) QA_CHECK(t[a].modcod_idx != t[b].modcod_idx);
+                 cusat_fec_info inf;
+                 QA_CHECK_EQ(cusat_fec_query(FECFRAME_NORMAL, t[a].code_rate, t[a].constellation, 0, &inf), CUSAT_OK);
+             }
          }},
     });
 }

@@ -103,7 +103,7 @@ MODCODS = [("QPSK", "1/2"), ("8PSK", "3/4"), ("8PSK", "3/5"), ("8PSK", "9/10"), 
            ("128APSK", "135/180"), ("256APSK", "135/180"), ("8APSK", "100/180"),
            ("16APSK", "90/180"), ("16APSK", "18/30"), ("32APSK", "2/3"), ("32APSK", "128/180"),
            ("64APSK", "132/180"), ("64APSK", "128/180"), ("128APSK", "140/180"),
-           ("256APSK", "116/180")]
+           ("256APSK", "116/180"), ("256APSK", "20/30"), ("256APSK", "22/30")]
 
 
 @pytest.mark.parametrize("mod,rate", MODCODS)
