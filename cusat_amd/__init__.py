@@ -91,6 +91,7 @@ def lib():
         L.cusat_bb_prbs.argtypes = [vp, i]
         L.cusat_bb_crc8.argtypes = [vp, i]
         L.cusat_encode_frames.argtypes = [i, i, i, ctypes.c_uint64, i, f, vp, vp, vp]
+        L.cusat_gen_frames_dev.argtypes = [vp, ctypes.c_uint64, i, f, vp, vp, vp, vp, vp]
         L.cusat_stream_create.argtypes = [ctypes.POINTER(vp), vp, i, i, u]
         L.cusat_stream_destroy.argtypes = [vp]
         L.cusat_stream_plan.argtypes = [vp, vp, i, vp]
@@ -190,6 +191,15 @@ class FecContext:
         _check(lib().cusat_ldpc_decode_learned(self._h, _ptr(llr), _ptr(hard), _ptr(trials), batch,
                                                max_trials, _stream(stream)),
                "cusat_ldpc_decode_learned")
+
+    def gen_frames_dev(self, seed, batch, es_n0_db, sym_out, info_in=None, info_out=None, cw_out=None,
+                       stream=None):
+        """Test traffic on the GPU (cusat_gen_frames_dev): device tensors
+        sym_out [B, xfec, 2] float32, optional info_in / info_out
+        [B, k_bch/8] uint8 and cw_out [B, n_ldpc/8] uint8."""
+        _check(lib().cusat_gen_frames_dev(self._h, int(seed) & 0xffffffffffffffff, batch, float(es_n0_db),
+                                          _ptr(info_in), _ptr(sym_out), _ptr(info_out), _ptr(cw_out),
+                                          _stream(stream)), "cusat_gen_frames_dev")
 
     def bb_extract(self, hard, trials, valid, out, dfl, batch, stream=None):
         """Post-BCH BB descramble + BBHEADER check + GSE data field (GPU)."""

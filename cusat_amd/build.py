@@ -21,7 +21,8 @@ ARCH = os.environ.get("CUSAT_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["codes.cc", "modulation.cc", "bch.cc", "gen.cc", "capi.cc",
            "ldpc_decode.hip", "ldpc_inst_a.hip", "ldpc_inst_b.hip", "ldpc_inst_c.hip",
-           "ldpc_inst_d.hip", "ldpc_inst_s.hip", "demap.hip", "bch_check.hip", "learn.hip", "ldpc_stream.hip", "bb.hip"]
+           "ldpc_inst_d.hip", "ldpc_inst_s.hip", "demap.hip", "bch_check.hip", "learn.hip", "ldpc_stream.hip", "bb.hip",
+           "gen_dev.hip"]
 
 
 def hipcc():

@@ -323,6 +323,7 @@ extern "C" int cusat_fec_ctx_destroy(cusat_fec_ctx* c)
     if (c->d_msg) hipFree(c->d_msg);
     if (c->d_mask) hipFree(c->d_mask);
     if (c->d_beta) hipFree(c->d_beta);
+    if (c->d_gen) hipFree(c->d_gen);
     delete c;
     return CUSAT_OK;
 }
@@ -427,9 +428,11 @@ extern "C" int cusat_ldpc_set_params(cusat_fec_ctx* c, const int16_t* beta, cons
         // beta block then alpha block, at the current capacity's layout
         c->d_alpha = c->d_beta + (size_t)c->param_cap * q;
         std::vector<int> h((size_t)c->param_cap * q * 2, 0);
+        // alpha word of the kernels: n | ceil(8096 / n) << 16 (0 = plain scale)
         for (size_t k = 0; k < n; k++) {
             h[k] = beta[k];
-            h[(size_t)c->param_cap * q + k] = alpha[k];
+            const int a = alpha[k];
+            h[(size_t)c->param_cap * q + k] = a ? (a | (((8096 + a - 1) / a) << 16)) : 0;
         }
         HIP_TRY(hipMemcpy(c->d_beta, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
     }

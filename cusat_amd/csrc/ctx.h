@@ -43,4 +43,9 @@ struct cusat_fec_ctx {
     int* d_beta = nullptr;
     int* d_alpha = nullptr;
     int n_param_iters = 0, param_cap = 0;
+    // test-traffic generator tables (cusat_gen_frames_dev), made on first use
+    void* d_gen = nullptr;
+    uint64_t* d_gen_pw = nullptr;
+    int *d_gen_off = nullptr, *d_gen_addr = nullptr;
+    int gen_chunk = 0;
 };

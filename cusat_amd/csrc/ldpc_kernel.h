@@ -89,7 +89,20 @@ __device__ __forceinline__ uint32_t pack_msg(s2 y_even, s2 y_odd)
     return __builtin_amdgcn_perm(bits(y_odd), bits(y_even), 0x06040200u);
 }
 
-__device__ __forceinline__ int nms_mag(int m, int n) { return (m * n + 32) >> 6; }
+// Scaled and clamped check-node magnitude min((m n + 32) >> 6, 127) of both
+// halves (m in 0..32767; the odd-symmetric fixed-point scale of
+// ldpc_decode_kernel.cu:75-90 with n in 64ths).  alpha_word = n |
+// ceil(8096 / n) << 16 (n in 1..1023, packed by the host): every m >= that
+// bound gives m n >= 8096, i.e. at least 127; below it m n + 32 < 8128, so
+// after clamping m to the bound the product is exact in 16-bit lanes and
+// four packed operations replace two 32-bit multiplies per value.
+__device__ __forceinline__ u2 nms_scale_clamp(u2 m, int alpha_word)
+{
+    const u2 n = (u2)(unsigned short)(alpha_word & 0xffff);
+    const u2 bound = (u2)(unsigned short)((unsigned)alpha_word >> 16);
+    const u2 mc = __builtin_elementwise_min(m, bound);
+    return __builtin_elementwise_min((u2)((mc * n + (u2)32) >> (u2)6), (u2)127);
+}
 
 // message record geometry for a kernel variant
 template <int CMAX>
@@ -387,13 +400,8 @@ __device__ __forceinline__ int decode_frame(unsigned char* lds_raw, const int16_
                     c1 = __builtin_elementwise_min(as_u2(mn1), (u2)127);
                     c2 = __builtin_elementwise_min(as_u2(mn2), (u2)127);
                 } else {
-                    s2 a1, a2;
-                    a1.x = (short)min(nms_mag(mn1.x, al), 127);
-                    a1.y = (short)min(nms_mag(mn1.y, al), 127);
-                    a2.x = (short)min(nms_mag(mn2.x, al), 127);
-                    a2.y = (short)min(nms_mag(mn2.y, al), 127);
-                    c1 = as_u2(a1);
-                    c2 = as_u2(a2);
+                    c1 = nms_scale_clamp(as_u2(mn1), al);
+                    c2 = nms_scale_clamp(as_u2(mn2), al);
                 }
                 // y = sign(extrinsic) * (m == min1 ? c2 : c1): m >= min1, so
                 // (min1 + 1) -| m is 1 exactly on the arg-min edge; its

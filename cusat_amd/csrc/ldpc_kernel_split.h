@@ -294,13 +294,8 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                     c1 = __builtin_elementwise_min(as_u2(b1), (u2)127);
                     c2 = __builtin_elementwise_min(as_u2(b2), (u2)127);
                 } else {
-                    s2 a1, a2;
-                    a1.x = (short)min(nms_mag(b1.x, al), 127);
-                    a1.y = (short)min(nms_mag(b1.y, al), 127);
-                    a2.x = (short)min(nms_mag(b2.x, al), 127);
-                    a2.y = (short)min(nms_mag(b2.y, al), 127);
-                    c1 = as_u2(a1);
-                    c2 = as_u2(a2);
+                    c1 = nms_scale_clamp(as_u2(b1), al);
+                    c2 = nms_scale_clamp(as_u2(b2), al);
                 }
                 __asm__("" : "+v"(sg));
                 // which edges write a delta slot: the same for every lane of the

@@ -89,8 +89,9 @@ def snr_scales(es_n0_db):
 
 
 class FramePool:
-    """Noiseless XFECFRAMEs of seeded random codewords, resident on the
-    device, plus their codeword bits; noisy LLR batches are drawn from it."""
+    """Noiseless XFECFRAMEs of seeded random codewords (host generator),
+    resident on the device, plus their codeword bits; noisy LLR batches are
+    drawn from it."""
 
     def __init__(self, ctx, framesize, rate, modulation, n_frames, seed):
         import torch
@@ -117,15 +118,53 @@ class FramePool:
             noisy = (s + noise).contiguous()
             self.ctx.demap(noisy, llr_out[k0:k1], k1 - k0, nv, gain)
 
+    def draw(self, idx, es_db, gen, llr_out, sub=32):
+        """Noisy frames of pool rows idx -> llr_out; returns their (info
+        bytes, codeword bytes) on the device."""
+        self.llr(idx, es_db, gen, llr_out, sub)
+        return self.info[idx], self.cw[idx]
+
+
+class GpuFrames:
+    """Fresh frames for every draw from the GPU transmit chain
+    (cusat_gen_frames_dev): new random codewords, AWGN at one Es/N0 per
+    sub-batch, demapped into llr_out -- no host generator in the loop."""
+
+    def __init__(self, ctx, max_batch, seed):
+        import torch
+        self.ctx = ctx
+        g = ctx.info
+        self.seed = int(seed)
+        self.calls = 0
+        self.sym = torch.empty((max_batch, g["xfec_len"], 2), dtype=torch.float32, device="cuda")
+        self.info = torch.empty((max_batch, g["k_bch"] // 8), dtype=torch.uint8, device="cuda")
+        self.cw = torch.empty((max_batch, g["n_ldpc"] // 8), dtype=torch.uint8, device="cuda")
+
+    def draw(self, B, es_db, llr_out, sub=32):
+        for k0 in range(0, B, sub):
+            k1 = min(B, k0 + sub)
+            es = float(es_db[k0 // sub])
+            nv, gain = snr_scales(es)
+            self.calls += 1
+            self.ctx.gen_frames_dev(self.seed * 1000003 + self.calls, k1 - k0, es, self.sym[k0:k1],
+                                    info_out=self.info[k0:k1], cw_out=self.cw[k0:k1])
+            self.ctx.demap(self.sym[k0:k1], llr_out[k0:k1], k1 - k0, nv, gain)
+        return self.info[:B], self.cw[:B]
+
 
 def train(framesize=0, rate="2/3", modulation="16APSK", n_iters=20, batch=512, steps=400,
           es_lo=8.9, es_hi=9.6, pool_frames=2048, seed=20240611, kappa=0.5, lr_alpha=0.005,
-          lr_beta=0.02, quantized=True, log=print):
+          lr_beta=0.02, quantized=True, log=print, frames="host"):
+    """frames: "host" = a pool of host-generated codewords re-noised per
+    batch; "gpu" = fresh codewords per batch from the GPU generator."""
     import torch
     import cusat_amd as ca
     ctx = ca.FecContext(framesize, rate, modulation, max_trials=50, max_batch=max(batch, 64))
     q, N = ctx.info["q"], ctx.info["n_ldpc"]
-    pool = FramePool(ctx, framesize, rate, modulation, pool_frames, seed)
+    if frames == "gpu":
+        pool = GpuFrames(ctx, batch, seed)
+    else:
+        pool = FramePool(ctx, framesize, rate, modulation, pool_frames, seed)
     lc = ca.LearnContext(ctx, batch, n_iters)
     dev = torch.device("cuda")
     alpha = torch.ones((n_iters, q), dtype=torch.float32, device=dev)
@@ -140,11 +179,15 @@ def train(framesize=0, rate="2/3", modulation="16APSK", n_iters=20, batch=512, s
     hist = []
     t0 = time.time()
     for step in range(1, steps + 1):
-        idx = torch.from_numpy(rng.choice(pool.n, batch, replace=False)).to(dev)
-        es = rng.uniform(es_lo, es_hi, (batch + 31) // 32)
-        pool.llr(idx, es, gen, llr16)
+        if frames == "gpu":
+            es = rng.uniform(es_lo, es_hi, (batch + 31) // 32)
+            _, cw = pool.draw(batch, es, llr16)
+        else:
+            idx = torch.from_numpy(rng.choice(pool.n, batch, replace=False)).to(dev)
+            es = rng.uniform(es_lo, es_hi, (batch + 31) // 32)
+            _, cw = pool.draw(idx, es, gen, llr16)
         llrf = llr16.float()
-        cw = pool.cw[idx].contiguous()
+        cw = cw.contiguous()
         ga.zero_(), gb.zero_(), loss.zero_()
         lc.fwd_bwd(llrf, cw, batch, alpha, beta, kappa, ga, gb, loss, quantized=quantized)
         ca.adam_step(alpha, ga, ma, va, lr_alpha, step, 0.25, 2.0)
@@ -164,17 +207,20 @@ def train(framesize=0, rate="2/3", modulation="16APSK", n_iters=20, batch=512, s
             "n_iters": n_iters, "alpha": a.round(5).tolist(), "beta": b.round(5).tolist(),
             "alpha_q": aq.tolist(), "beta_q": bq.tolist(),
             "training": {"batch": batch, "steps": steps, "es_n0_db": [es_lo, es_hi],
-                         "pool_frames": pool_frames, "seed": seed, "kappa": kappa,
+                         "pool_frames": pool_frames if frames == "host" else None, "frames": frames,
+                         "seed": seed, "kappa": kappa,
                          "lr_alpha": lr_alpha, "lr_beta": lr_beta, "quantized": quantized,
                          "history": hist}}
 
 
 def evaluate(params_path=DEFAULT_PARAMS, points=(8.8, 9.0, 9.2, 9.4, 9.6), frames=16384, chunk=1024,
-             learned_trials=20, plain_trials=50, pool_frames=2048, seed=777, log=print, params=None):
+             learned_trials=20, plain_trials=50, pool_frames=2048, seed=777, log=print, params=None,
+             source="host"):
     """FER and decoded info Gbit/s per Es/N0 of: the plain decoder (T = 50),
     the learned decoder alone (T = learned_trials) and the learned decoder
     with plain fallback, on the same noisy LLRs.  `params`: a parameter dict
-    as train() returns, instead of the file."""
+    as train() returns, instead of the file.  source: "host" (re-noised
+    pool of host-generated codewords) or "gpu" (fresh GPU-generated frames)."""
     import torch
     import cusat_amd as ca
     if params is None:
@@ -192,7 +238,7 @@ def evaluate(params_path=DEFAULT_PARAMS, points=(8.8, 9.0, 9.2, 9.4, 9.6), frame
     ctx_id.set_params(np.ones_like(bq), np.full_like(aq, 64))
     inf = ctx.info
     N, K, kb = inf["n_ldpc"], inf["k_ldpc"], inf["k_bch"]
-    pool = FramePool(ctx, fs, rate, mod, pool_frames, seed)
+    pool = GpuFrames(ctx, chunk, seed) if source == "gpu" else FramePool(ctx, fs, rate, mod, pool_frames, seed)
     dev = torch.device("cuda")
     gen = torch.Generator(device=dev)
     gen.manual_seed(seed)
@@ -208,9 +254,12 @@ def evaluate(params_path=DEFAULT_PARAMS, points=(8.8, 9.0, 9.2, 9.4, 9.6), frame
         acc = {m: {"err": 0, "ldpc_fail": 0, "iters": 0, "ms": 0.0, "fallback": 0}
                for m in ("plain", "learned", "learned+fallback", "identity+fallback")}
         for c0 in range(0, frames, chunk):
-            idx = torch.from_numpy(rng.integers(0, pool.n, chunk)).to(dev)
-            pool.llr(idx, np.full((chunk + 31) // 32, es), gen, llr)
-            info_ref = pool.info[idx]
+            es_sub = np.full((chunk + 31) // 32, es)
+            if source == "gpu":
+                info_ref, _ = pool.draw(chunk, es_sub, llr)
+            else:
+                info_ref, _ = pool.draw(torch.from_numpy(rng.integers(0, pool.n, chunk)).to(dev), es_sub, gen, llr)
+            info_ref = info_ref.clone()
             for m in acc:
                 torch.cuda.synchronize()
                 ev[0].record()
@@ -250,7 +299,7 @@ def evaluate(params_path=DEFAULT_PARAMS, points=(8.8, 9.0, 9.2, 9.4, 9.6), frame
 
 
 def evaluate_stream(params_path=DEFAULT_PARAMS, points=(9.0, 9.2, 9.4), frames=32768, learned_trials=20,
-                    plain_trials=50, pool_frames=2048, seed=777, reps=3, log=print):
+                    plain_trials=50, pool_frames=2048, seed=777, reps=3, log=print, source="host"):
     """The plain decoder vs the learned decoder with plain fallback as a
     receiver runs them on a continuous stream: all `frames` frames of a
     point (LLRs resident in HBM) decoded by ONE persistent launch
@@ -268,7 +317,7 @@ def evaluate_stream(params_path=DEFAULT_PARAMS, points=(9.0, 9.2, 9.4), frames=3
     ctx.set_params(bq, aq)
     inf = ctx.info
     N, K, kb = inf["n_ldpc"], inf["k_ldpc"], inf["k_bch"]
-    pool = FramePool(ctx, fs, rate, mod, pool_frames, seed)
+    pool = GpuFrames(ctx, chunk, seed) if source == "gpu" else FramePool(ctx, fs, rate, mod, pool_frames, seed)
     dev = torch.device("cuda")
     gen = torch.Generator(device=dev)
     gen.manual_seed(seed)
@@ -288,10 +337,15 @@ def evaluate_stream(params_path=DEFAULT_PARAMS, points=(9.0, 9.2, 9.4), frames=3
         sc.plan(plan)
     rows = []
     for es in points:
-        idx = torch.from_numpy(rng.integers(0, pool.n, frames)).to(dev)
+        info_ref = torch.empty((frames, kb // 8), dtype=torch.uint8, device=dev)
+        es_sub = np.full((chunk + 31) // 32, es)
+        idx = None if source == "gpu" else torch.from_numpy(rng.integers(0, pool.n, frames)).to(dev)
         for c0 in range(0, frames, chunk):
-            pool.llr(idx[c0:c0 + chunk], np.full((chunk + 31) // 32, es), gen, llr[c0:c0 + chunk])
-        info_ref = pool.info[idx]
+            if source == "gpu":
+                inf, _ = pool.draw(chunk, es_sub, llr[c0:c0 + chunk])
+            else:
+                inf, _ = pool.draw(idx[c0:c0 + chunk], es_sub, gen, llr[c0:c0 + chunk])
+            info_ref[c0:c0 + chunk] = inf
         for m, sc in streams.items():
             sc.decode(llr, hard, tr, va, fb)
             torch.cuda.synchronize()
@@ -401,6 +455,8 @@ def main(argv=None):
     t.add_argument("--lr-beta", type=float, default=0.02)
     t.add_argument("--float-model", action="store_true", help="train the fp32 model without rounding")
     t.add_argument("--es", default="8.9,9.6", help="training Es/N0 range in dB")
+    t.add_argument("--gen", default="host", choices=["host", "gpu"],
+                   help="training frames: re-noised host-generated pool, or fresh GPU-generated frames")
     ts = sub.add_parser("train-set")
     ts.add_argument("--codes", default="s2x", choices=["mixed12", "s2x"])
     ts.add_argument("--steps", type=int, default=200)
@@ -417,12 +473,15 @@ def main(argv=None):
     e.add_argument("--out", default=None)
     e.add_argument("--stream", action="store_true",
                    help="one persistent launch per point over all frames (plain vs learned + fallback)")
+    e.add_argument("--gen", default="host", choices=["host", "gpu"],
+                   help="frames: re-noised pool of host-generated codewords, or fresh GPU-generated ones")
     a = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)
     if a.cmd == "train":
         lo, hi = [float(x) for x in a.es.split(",")]
         d = train(n_iters=a.iters, batch=a.batch, steps=a.steps, pool_frames=a.pool, es_lo=lo, es_hi=hi,
-                  lr_alpha=a.lr_alpha, lr_beta=a.lr_beta, quantized=not a.float_model, log=log)
+                  lr_alpha=a.lr_alpha, lr_beta=a.lr_beta, quantized=not a.float_model, log=log,
+                  frames=a.gen)
         os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
         with open(a.out, "w") as f:
             json.dump(d, f, indent=1)
@@ -433,9 +492,9 @@ def main(argv=None):
     else:
         pts = [float(x) for x in a.points.split(",")]
         if a.stream:
-            rows = evaluate_stream(a.params, pts, a.frames, log=log)
+            rows = evaluate_stream(a.params, pts, a.frames, log=log, source=a.gen)
         else:
-            rows = evaluate(a.params, pts, a.frames, log=log)
+            rows = evaluate(a.params, pts, a.frames, log=log, source=a.gen)
         summary = {m: fer_crossing(rows, m) for m in sorted(set(r["decoder"] for r in rows))}
         log(json.dumps({"fer_1e-3_crossing": summary}))
         if a.out:

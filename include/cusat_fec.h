@@ -282,6 +282,22 @@ int cusat_gen_frames(int framesize, int rate, int modulation, uint64_t seed, int
 int cusat_encode_frames(int framesize, int rate, int modulation, uint64_t seed, int batch,
                         float es_n0_db, const uint8_t* info_in, float* sym_out, uint8_t* cw_out);
 
+/* ---- test-traffic generator (GPU) ----
+ * The transmit chain of cusat_gen_frames on the device, one workgroup per
+ * frame, for the context's (frame size, rate, modulation): info bytes
+ * (info_in, B x k_bch/8 device bytes, or NULL for counter-based random bytes
+ * of (seed, frame)) -> BCH -> IRA LDPC -> interleave + map -> AWGN at
+ * es_n0_db (none when >= 1000; counter-based Gaussians of (seed, frame,
+ * symbol)).  sym_out = B x xfec_len complex float (device); info_out
+ * (B x k_bch/8) and cw_out (B x n_ldpc/8) device bytes or NULL.  Codewords
+ * and noiseless symbols equal cusat_encode_frames' for the same info bytes;
+ * random bytes and noise are not the host generator's streams.  Replaces
+ * the host encode of the reference's test chain (ldpc_encode_kernel.cu:1-17,
+ * mapper_kernel.cu:22-51, apps/bench_coded_modulation.cc:205-360). */
+int cusat_gen_frames_dev(cusat_fec_ctx* ctx, uint64_t seed, int batch, float es_n0_db,
+                         const uint8_t* info_in, float* sym_out, uint8_t* info_out, uint8_t* cw_out,
+                         void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -146,3 +146,17 @@ def test_trained_parameter_files():
     want = {(fs, r, m) for r, m, _ in learn.MODCODS12 + learn.S2X
             for fs in ((0, 1) if learn.has_short(r, m) else (0,))}
     assert want <= seen
+
+
+def test_packed_alpha_scale_identity():
+    """The decoders evaluate min((m n + 32) >> 6, 127) in 16-bit lanes after
+    clamping m to ceil(8096 / n) (alpha word n | bound << 16, packed by
+    cusat_ldpc_set_params): exhaustively equal to the 32-bit form for every
+    scale n in 1..1023 and magnitude m in 0..32767."""
+    m = np.arange(32768, dtype=np.int64)
+    for n in range(1, 1024):
+        ref = np.minimum((m * n + 32) >> 6, 127)
+        bound = (8096 + n - 1) // n
+        mc = np.minimum(m, bound)
+        assert (mc * n + 32).max() < 1 << 16
+        np.testing.assert_array_equal(np.minimum(((mc * n + 32) & 0xffff) >> 6, 127), ref)
