@@ -55,12 +55,13 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
     uint32_t* par32 = app32 + p.K / 2;                        // R/2 dwords
     uint32_t* dsl32 = par32 + p.R / 2;                        // n_slots * 180 dwords
     uint32_t* xch = dsl32 + p.n_slots * HALF;                 // [3][LANES][2][180] partials
-    // per (step, lane): layer (-1 idle), its lay_off and data-circulant count,
-    // so that a step's first global access is its circulant records
+    // per (step, lane): layer (-1 idle), its lay_off and data-circulant
+    // count, and the step's first collision group, so that a step's first
+    // global access is its circulant records
     int4* sched_l = reinterpret_cast<int4*>(xch + 3 * LANES * 2 * HALF);
-    int* goff_l = reinterpret_cast<int*>(sched_l + 2 * p.n_steps);
-    int* grp_l = goff_l + p.n_steps + 1;
-    int* flg = grp_l + GROUP_WORDS * p.n_groups;  // [3] wg_any flags
+    int* grp_l = reinterpret_cast<int*>(sched_l + 2 * p.n_steps);  // [n_groups][8], 16-byte aligned
+    int* goff_l = grp_l + GROUP_WORDS * p.n_groups;
+    int* flg = goff_l + p.n_steps + 1;  // [3] wg_any flags
     int16_t* app16 = reinterpret_cast<int16_t*>(lds_raw);
     int16_t* par16 = reinterpret_cast<int16_t*>(par32);
 
@@ -82,12 +83,13 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
     for (int k = t; k < 2 * p.n_steps; k += NTHR) {
         const int i = steps[k];
         const int ic = max(i, 0);
-        sched_l[k] = make_int4(i, lay_off[ic], EXACT ? CMAX : lay_cnt[ic], 0);
+        sched_l[k] = make_int4(i, lay_off[ic], EXACT ? CMAX : lay_cnt[ic], grp_off[k >> 1]);
     }
     for (int k = t; k <= p.n_steps; k += NTHR) goff_l[k] = grp_off[k];
     for (int k = t; k < GROUP_WORDS * p.n_groups; k += NTHR) grp_l[k] = groups[k];
     if (t < 3) flg[t] = 0;
     int any_k = 0;
+    const int n_groups_all = p.n_groups;  // first group past the last step
     // input clamp of the int16 APP (ldpc_decode_kernel.cu:655-670): -32768 -> -32767
     for (int v = t; v < p.K; v += NTHR) app16[vn_index(v)] = (int16_t)max((int)llr[v], -32767);
     for (int c = t; c < p.R; c += NTHR) {
@@ -125,15 +127,17 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
     };
     uint4 ra[R::V4], rb[R::V4];
     int pb[2], pa[2];  // beta, alpha of the step whose record is in ra / rb
-    int si[2], so[2], sc[2];  // its layer, lay_off and circulant count
+    int si[2], so[2], sc[2], sg0[2];  // its layer, lay_off, circulant count, first group
     {
         const int4 e0 = sched_l[lane];
         si[0] = __builtin_amdgcn_readfirstlane(e0.x);
         so[0] = __builtin_amdgcn_readfirstlane(e0.y);
         sc[0] = __builtin_amdgcn_readfirstlane(e0.z);
+        sg0[0] = __builtin_amdgcn_readfirstlane(e0.w);
         si[1] = -1;
         so[1] = 0;
         sc[1] = 0;
+        sg0[1] = 0;
         const int i0 = max(si[0], 0);
         const uint4* src = rec_at(i0, tl);
 #pragma unroll
@@ -206,6 +210,7 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                 si[hs ^ 1] = __builtin_amdgcn_readfirstlane(en.x);
                 so[hs ^ 1] = __builtin_amdgcn_readfirstlane(en.y);
                 sc[hs ^ 1] = __builtin_amdgcn_readfirstlane(en.z);
+                sg0[hs ^ 1] = __builtin_amdgcn_readfirstlane(en.w);
                 const int inx = max(si[hs ^ 1], 0);
                 const uint4* src = rec_at(inx, tl);
 #pragma unroll
@@ -351,19 +356,35 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                     dst[k] = make_uint4(ow[4 * k], ow[4 * k + 1], ow[4 * k + 2], ow[4 * k + 3]);
             }
             __syncthreads();
-            const int g0 = __builtin_amdgcn_readfirstlane(goff_l[s]);
-            const int g1 = __builtin_amdgcn_readfirstlane(goff_l[s + 1]);
+            // collision groups of the step: [first group of this step, first
+            // of the next) -- the next step's entry was read at the prefetch
+            const int g0 = sg0[hs];
+            const int g1 = s + 1 < p.n_steps ? sg0[hs ^ 1] : n_groups_all;
             if (g1 > g0) {
+                // one group per wave group at a time; thread tt adds the deltas
+                // of pair dword tt in slot order.  The group record comes in
+                // two 16-byte reads and every slot read is issued before the
+                // first add, so a group costs two LDS round trips.
                 for (int gb = g0; gb < g1; gb += 2 * LANES) {
                     const int g = gb + quarter;
                     if (g < g1 && act) {
-                        const int* gr = grp_l + GROUP_WORDS * g;
-                        const int base = __builtin_amdgcn_readfirstlane(gr[0]);
-                        const int n = __builtin_amdgcn_readfirstlane(gr[1]);
-                        s2 val = as_s2(app32[base + tt]);
-                        for (int e = 0; e < n; e++)
-                            val = __builtin_elementwise_add_sat(
-                                val, as_s2(app32[__builtin_amdgcn_readfirstlane(gr[2 + e]) + tt]));
+                        const int4 ra4 = *reinterpret_cast<const int4*>(grp_l + GROUP_WORDS * g);
+                        const int4 rb4 = *reinterpret_cast<const int4*>(grp_l + GROUP_WORDS * g + 4);
+                        const int base = __builtin_amdgcn_readfirstlane(ra4.x);
+                        const int n = __builtin_amdgcn_readfirstlane(ra4.y);
+                        const int sl[6] = {__builtin_amdgcn_readfirstlane(ra4.z), __builtin_amdgcn_readfirstlane(ra4.w),
+                                           __builtin_amdgcn_readfirstlane(rb4.x), __builtin_amdgcn_readfirstlane(rb4.y),
+                                           __builtin_amdgcn_readfirstlane(rb4.z), __builtin_amdgcn_readfirstlane(rb4.w)};
+                        const uint32_t a0 = app32[base + tt];
+                        // all six read unconditionally (a record's unused slot
+                        // entries are 0: in-bounds reads that are never added)
+                        uint32_t dl[6];
+#pragma unroll
+                        for (int e = 0; e < 6; e++) dl[e] = app32[sl[e] + tt];
+                        s2 val = as_s2(a0);
+#pragma unroll
+                        for (int e = 0; e < 6; e++)
+                            if (e < n) val = __builtin_elementwise_add_sat(val, as_s2(dl[e]));
                         app32[base + tt] = bits(val);
                     }
                 }
