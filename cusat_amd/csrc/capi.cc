@@ -78,6 +78,8 @@ int upload_tables(cusat_fec_ctx* c)
         grp_off.push_back(grp_off.back());
     }
     c->kern_steps = (int)steps32.size() / 2;
+    std::vector<int> defer32(c->kern_steps, 0);
+    for (int st = 0; st < g.n_steps; st++) defer32[st] = g.step_defer[st];
     // collision groups as int32 records: LDS dword base of the VN group,
     // slot count, slot dword bases (slots follow app + parity in LDS)
     std::vector<int> grp32;
@@ -102,6 +104,7 @@ int upload_tables(cusat_fec_ctx* c)
         {g.lay_off.data(), g.lay_off.size() * 4, (void**)&c->d_lay_off},
         {g.lay_moff.data(), g.lay_moff.size() * 4, (void**)&c->d_lay_moff},
         {steps32.data(), steps32.size() * 4, (void**)&c->d_steps},
+        {defer32.data(), defer32.size() * 4, (void**)&c->d_step_defer},
         {grp_off.data(), grp_off.size() * 4, (void**)&c->d_grp_off},
         {grp32.data(), grp32.size() * 4, (void**)&c->d_groups},
         {c->con.xy.data(), (size_t)c->con.n_points * 2 * 4, (void**)&c->d_pts},
@@ -138,6 +141,7 @@ ldpc_args make_ldpc_args(const cusat_fec_ctx* c, const int16_t* llr, uint8_t* ha
     a.lay_moff = c->d_lay_moff;
     a.steps = c->d_steps;
     a.grp_off = c->d_grp_off;
+    a.step_defer = c->d_step_defer;
     a.groups = c->d_groups;
     a.N = c->geo.N;
     a.K = c->geo.K;
@@ -217,7 +221,14 @@ int prepare(cusat_fec_ctx* c, int framesize, int rate, int modulation, unsigned 
         return CUSAT_E_INVALID;
     }
     try {
-        c->geo = build_geometry(*tab, !(flags & CUSAT_CTX_SEQUENTIAL_SCHEDULE));
+        const bool sched = !(flags & CUSAT_CTX_SEQUENTIAL_SCHEDULE);
+        c->geo = build_geometry(*tab, sched, true);
+        // deferred collision groups need a second slot area: keep them only
+        // where both decoders' LDS still fits in 160 KiB
+        const int ks = c->geo.n_steps + (c->geo.n_steps & 1), ng = (int)c->geo.groups.size();
+        const int lds_max = std::max(ldpc_lds_bytes(c->geo.K, c->geo.R, c->geo.n_slots, ks, ng),
+                                     ldpc_lds_bytes_split(c->geo.K, c->geo.R, c->geo.n_slots, ks, ng, 2));
+        if (c->geo.slot_parity_base > 0 && lds_max > 160 * 1024) c->geo = build_geometry(*tab, sched, false);
     } catch (const std::exception& e) {
         set_last_error("geometry: %s", e.what());
         return CUSAT_E_INVALID;

@@ -8,6 +8,7 @@
 // ldpc_table_factory.cc:17-95.
 #include "codes.h"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -185,7 +186,7 @@ std::vector<std::array<int, 2>> find_schedule(const std::string& table_name, int
     return it->second;
 }
 
-decoder_geometry build_geometry(const ldpc_table& t, bool use_schedule)
+decoder_geometry build_geometry(const ldpc_table& t, bool use_schedule, bool defer_groups)
 {
     decoder_geometry g;
     g.N = t.N;
@@ -245,17 +246,43 @@ decoder_geometry build_geometry(const ldpc_table& t, bool use_schedule)
 
     g.circ.resize(4 * (size_t)off);
     g.grp_off.push_back(0);
-    for (int s = 0; s < g.n_steps; s++) {
-        // all (lane, k) circulants of the step in application order
-        std::map<int, std::vector<int>> by_row;  // row -> global circulant ids
+    const int S = g.n_steps;
+    // all (lane, k) circulants of every step in application order, by row
+    std::vector<std::map<int, std::vector<int>>> by_row(S);
+    for (int s = 0; s < S; s++)
         for (int l = 0; l < 2; l++) {
             int i = sched[s][l];
             if (i < 0) continue;
-            for (int k = 0; k < g.lay_cnt[i]; k++) by_row[lay[i][k].first].push_back(g.lay_off[i] + k);
+            for (int k = 0; k < g.lay_cnt[i]; k++) by_row[s][lay[i][k].first].push_back(g.lay_off[i] + k);
         }
-        int slot = 0;
+    // Deferred collision groups: a step whose collided rows the next step
+    // does not touch may leave its groups to be applied during the next
+    // step (no extra barrier); the two steps then use disjoint slot areas
+    // (slot numbers offset by step parity).  The last step of an iteration
+    // is never deferred: the syndrome check that follows reads every row.
+    std::vector<int> n_slot(S, 0);
+    g.step_defer.assign(S, 0);
+    bool any_defer = false;
+    for (int s = 0; s < S; s++) {
+        bool has = false, ok = s + 1 < S;
+        for (auto& kv : by_row[s]) {
+            if (kv.second.size() < 2) continue;
+            has = true;
+            n_slot[s] += (int)kv.second.size();
+            if (ok && by_row[s + 1].count(kv.first)) ok = false;
+        }
+        g.step_defer[s] = (defer_groups && has && ok) ? 1 : 0;
+        any_defer |= g.step_defer[s] != 0;
+    }
+    int max_even = 0, max_odd = 0;
+    for (int s = 0; s < S; s++) (s & 1 ? max_odd : max_even) = std::max(s & 1 ? max_odd : max_even, n_slot[s]);
+    g.slot_parity_base = any_defer ? max_even : 0;
+    g.n_slots = any_defer ? max_even + max_odd : std::max(max_even, max_odd);
+
+    for (int s = 0; s < S; s++) {
+        int slot = any_defer && (s & 1) ? max_even : 0;
         std::map<int, int> slot_of;
-        for (auto& kv : by_row) {
+        for (auto& kv : by_row[s]) {
             if (kv.second.size() < 2) continue;
             if (kv.second.size() > 6) throw std::runtime_error("collision group too large");
             collision_group cg{};
@@ -268,7 +295,6 @@ decoder_geometry build_geometry(const ldpc_table& t, bool use_schedule)
             g.groups.push_back(cg);
         }
         if (slot > 63) throw std::runtime_error("too many collision slots in a step");
-        if (slot > g.n_slots) g.n_slots = slot;
         for (int l = 0; l < 2; l++) {
             int i = sched[s][l];
             if (i < 0) continue;

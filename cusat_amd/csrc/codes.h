@@ -60,10 +60,15 @@ struct decoder_geometry {
     std::vector<int16_t> steps;  // [n_steps][2], -1 = idle lane
     std::vector<int> grp_off;    // [n_steps + 1]
     std::vector<collision_group> groups;
+    // [n_steps] 1: the step's collision groups may be applied during the
+    // next step, which touches none of their rows (no barrier of their own)
+    std::vector<int> step_defer;
+    int slot_parity_base = 0;   // first slot of odd steps (0: one slot area)
 };
 
 // Per-layer circulant lists in table order (row asc, address index asc),
-// schedule, and the collision plan of every step.
-decoder_geometry build_geometry(const ldpc_table& t, bool use_schedule);
+// schedule, and the collision plan of every step.  defer_groups: plan
+// deferred collision groups (disjoint slot areas for consecutive steps).
+decoder_geometry build_geometry(const ldpc_table& t, bool use_schedule, bool defer_groups = true);
 
 }  // namespace cusat

@@ -32,6 +32,7 @@ struct cusat_fec_ctx {
     uint32_t* d_circ = nullptr;
     int *d_lay_cnt = nullptr, *d_lay_off = nullptr, *d_lay_moff = nullptr, *d_grp_off = nullptr;
     int* d_steps = nullptr;
+    int* d_step_defer = nullptr;  // [kern_steps] deferred collision groups (1) per step
     int kern_steps = 0;  // schedule steps as launched (padded to even)
     int* d_groups = nullptr;
     float* d_pts = nullptr;

@@ -21,6 +21,7 @@ struct ldpc_args {
     const int* steps;          // [n_steps][2], n_steps even (padded with idle steps)
     const int* grp_off;        // [n_steps + 1]
     const int* groups;         // [n_groups][8] int32 collision group records
+    const int* step_defer;     // [n_steps] 1 = the step's groups are applied during the next step
     const int* beta;           // [n_param_iters][q] offsets, or null (plain: 1)
     const int* alpha;          // [n_param_iters][q] scales in 64ths, or null (off)
     int n_param_iters;

@@ -40,12 +40,37 @@ struct rec_half_t {
     static constexpr int V4 = WORDS / 4;
 };
 
+// One collision group at pair dword tt: APP = sat(...sat(APP + d_1)... + d_n)
+// in slot order.  The group record comes in two 16-byte reads; a record's
+// unused slot entries are 0 (in-bounds reads never added), so all six slot
+// reads are issued before the first add: two LDS round trips per group.
+__device__ __forceinline__ void group_apply(uint32_t* app32, const int* rec, int tt)
+{
+    const int4 ra4 = *reinterpret_cast<const int4*>(rec);
+    const int4 rb4 = *reinterpret_cast<const int4*>(rec + 4);
+    const int base = __builtin_amdgcn_readfirstlane(ra4.x);
+    const int n = __builtin_amdgcn_readfirstlane(ra4.y);
+    const int sl[6] = {__builtin_amdgcn_readfirstlane(ra4.z), __builtin_amdgcn_readfirstlane(ra4.w),
+                       __builtin_amdgcn_readfirstlane(rb4.x), __builtin_amdgcn_readfirstlane(rb4.y),
+                       __builtin_amdgcn_readfirstlane(rb4.z), __builtin_amdgcn_readfirstlane(rb4.w)};
+    const uint32_t a0 = app32[base + tt];
+    uint32_t dl[6];
+#pragma unroll
+    for (int e = 0; e < 6; e++) dl[e] = app32[sl[e] + tt];
+    s2 val = as_s2(a0);
+#pragma unroll
+    for (int e = 0; e < 6; e++)
+        if (e < n) val = __builtin_elementwise_add_sat(val, as_s2(dl[e]));
+    app32[base + tt] = bits(val);
+}
+
 template <int CMAX, bool EXACT, int LANES>
 __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const int16_t* __restrict__ llr,
                                                   uint4* __restrict__ rec_base, uint8_t* __restrict__ h,
                                                   const circ_rec* __restrict__ circ,
                                                   const int* __restrict__ lay_cnt, const int* __restrict__ lay_off,
                                                   const int* __restrict__ steps, const int* __restrict__ grp_off,
+                                                  const int* __restrict__ step_defer,
                                                   const int* __restrict__ groups, const int* __restrict__ beta,
                                                   const int* __restrict__ alpha, const kparams& p)
 {
@@ -83,7 +108,9 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
     for (int k = t; k < 2 * p.n_steps; k += NTHR) {
         const int i = steps[k];
         const int ic = max(i, 0);
-        sched_l[k] = make_int4(i, lay_off[ic], EXACT ? CMAX : lay_cnt[ic], grp_off[k >> 1]);
+        // .w: first collision group of the step | deferred-groups flag << 30
+        sched_l[k] = make_int4(i, lay_off[ic], EXACT ? CMAX : lay_cnt[ic],
+                               grp_off[k >> 1] | (step_defer[k >> 1] ? 1 << 30 : 0));
     }
     for (int k = t; k <= p.n_steps; k += NTHR) goff_l[k] = grp_off[k];
     for (int k = t; k < GROUP_WORDS * p.n_groups; k += NTHR) grp_l[k] = groups[k];
@@ -133,7 +160,7 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
         si[0] = __builtin_amdgcn_readfirstlane(e0.x);
         so[0] = __builtin_amdgcn_readfirstlane(e0.y);
         sc[0] = __builtin_amdgcn_readfirstlane(e0.z);
-        sg0[0] = __builtin_amdgcn_readfirstlane(e0.w);
+        sg0[0] = __builtin_amdgcn_readfirstlane(e0.w);  // step 0
         si[1] = -1;
         so[1] = 0;
         sc[1] = 0;
@@ -186,6 +213,10 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
         if (it == p.max_trials) break;
 
         // ---- one layered iteration ----
+        // collision groups of the previous step left to the current one
+        // (none at an iteration's start: its last step is never deferred)
+        int prev_g0 = 0, prev_g1 = 0;
+        bool prev_def = false;
         for (int s0 = 0; s0 < p.n_steps; s0 += 2) {
 #pragma unroll
         for (int hs = 0; hs < 2; hs++) {
@@ -235,6 +266,29 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
             // placeholder that cannot change min1/min2 or the sign
             const int pidx = half == 0 ? i * HALF + tt
                                        : (i > 0 ? (i - 1) * HALF + tt : (q - 1) * HALF + (tt > 0 ? tt - 1 : HALF - 1));
+            // Deferred groups of the previous step: their rows are untouched by
+            // this step, so wave group `quarter` applies group prev_g0 + quarter
+            // alongside the step -- reads issued here, the result written
+            // before the exchange barrier (more groups than wave groups: the
+            // rest after it, synchronously).
+            const bool mat = prev_def && prev_g0 + quarter < prev_g1;  // wave-uniform
+            int m_base = 0, m_n = 0;
+            uint32_t m_a0 = 0, m_d[6] = {0, 0, 0, 0, 0, 0};
+            if (mat) {
+                const int g = prev_g0 + quarter;
+                const int4 ga = *reinterpret_cast<const int4*>(grp_l + GROUP_WORDS * g);
+                const int4 gb = *reinterpret_cast<const int4*>(grp_l + GROUP_WORDS * g + 4);
+                m_base = __builtin_amdgcn_readfirstlane(ga.x);
+                m_n = __builtin_amdgcn_readfirstlane(ga.y);
+                const int sl[6] = {__builtin_amdgcn_readfirstlane(ga.z), __builtin_amdgcn_readfirstlane(ga.w),
+                                   __builtin_amdgcn_readfirstlane(gb.x), __builtin_amdgcn_readfirstlane(gb.y),
+                                   __builtin_amdgcn_readfirstlane(gb.z), __builtin_amdgcn_readfirstlane(gb.w)};
+                if (act) {
+                    m_a0 = app32[m_base + tt];
+#pragma unroll
+                    for (int e = 0; e < 6; e++) m_d[e] = app32[sl[e] + tt];
+                }
+            }
             const bool dummy = half == 1 && i == 0 && tt == 0;
             s2 xp = zero, mp = zero;
             const circ_rec* cc = circ + loff + half * n0;
@@ -278,7 +332,17 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                 xm2[xo] = bits(mn2);
                 xsg[xo] = sg;
             }
+            if (mat && act) {
+                s2 val = as_s2(m_a0);
+#pragma unroll
+                for (int e = 0; e < 6; e++)
+                    if (e < m_n) val = __builtin_elementwise_add_sat(val, as_s2(m_d[e]));
+                app32[m_base + tt] = bits(val);
+            }
             __syncthreads();  // every read of the step is done; partials are in LDS
+            if (prev_def && act)
+                for (int g = prev_g0 + quarter + 2 * LANES; g < prev_g1; g += 2 * LANES)
+                    group_apply(app32, grp_l + GROUP_WORDS * g, tt);
             if (work) {
                 {
                     const int po = (quarter ^ 1) * HALF + tt;  // the other half of the same lane
@@ -358,35 +422,20 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
             __syncthreads();
             // collision groups of the step: [first group of this step, first
             // of the next) -- the next step's entry was read at the prefetch
-            const int g0 = sg0[hs];
-            const int g1 = s + 1 < p.n_steps ? sg0[hs ^ 1] : n_groups_all;
-            if (g1 > g0) {
+            const int g0 = sg0[hs] & 0x3fffffff;
+            const bool defer = (sg0[hs] >> 30) & 1;
+            const int g1 = s + 1 < p.n_steps ? (sg0[hs ^ 1] & 0x3fffffff) : n_groups_all;
+            prev_g0 = g0;
+            prev_g1 = g1;
+            prev_def = defer && g1 > g0;
+            if (g1 > g0 && !defer) {
                 // one group per wave group at a time; thread tt adds the deltas
                 // of pair dword tt in slot order.  The group record comes in
                 // two 16-byte reads and every slot read is issued before the
                 // first add, so a group costs two LDS round trips.
                 for (int gb = g0; gb < g1; gb += 2 * LANES) {
                     const int g = gb + quarter;
-                    if (g < g1 && act) {
-                        const int4 ra4 = *reinterpret_cast<const int4*>(grp_l + GROUP_WORDS * g);
-                        const int4 rb4 = *reinterpret_cast<const int4*>(grp_l + GROUP_WORDS * g + 4);
-                        const int base = __builtin_amdgcn_readfirstlane(ra4.x);
-                        const int n = __builtin_amdgcn_readfirstlane(ra4.y);
-                        const int sl[6] = {__builtin_amdgcn_readfirstlane(ra4.z), __builtin_amdgcn_readfirstlane(ra4.w),
-                                           __builtin_amdgcn_readfirstlane(rb4.x), __builtin_amdgcn_readfirstlane(rb4.y),
-                                           __builtin_amdgcn_readfirstlane(rb4.z), __builtin_amdgcn_readfirstlane(rb4.w)};
-                        const uint32_t a0 = app32[base + tt];
-                        // all six read unconditionally (a record's unused slot
-                        // entries are 0: in-bounds reads that are never added)
-                        uint32_t dl[6];
-#pragma unroll
-                        for (int e = 0; e < 6; e++) dl[e] = app32[sl[e] + tt];
-                        s2 val = as_s2(a0);
-#pragma unroll
-                        for (int e = 0; e < 6; e++)
-                            if (e < n) val = __builtin_elementwise_add_sat(val, as_s2(dl[e]));
-                        app32[base + tt] = bits(val);
-                    }
+                    if (g < g1 && act) group_apply(app32, grp_l + GROUP_WORDS * g, tt);
                 }
                 __syncthreads();
             }
@@ -410,15 +459,17 @@ __global__ void __launch_bounds__(LANES* SPLIT_LANE_THREADS)
                       const circ_rec* __restrict__ circ, const int* __restrict__ lay_cnt,
                       const int* __restrict__ lay_off,
                       const int* __restrict__ steps, const int* __restrict__ grp_off,
-                      const int* __restrict__ groups, const int* __restrict__ beta,
-                      const int* __restrict__ alpha, const int32_t* __restrict__ active, kparams p)
+                      const int* __restrict__ step_defer, const int* __restrict__ groups,
+                      const int* __restrict__ beta, const int* __restrict__ alpha,
+                      const int32_t* __restrict__ active, kparams p)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int cw = blockIdx.x;
     if (active && active[cw] == 0) return;
     const int res = decode_frame_split<CMAX, EXACT, LANES>(
         lds_raw, llr_all + (size_t)cw * p.N, msg_all + (size_t)cw * p.q * 2 * HALF * rec_half_t<CMAX>::V4,
-        hard_all + (size_t)cw * (p.K >> 3), circ, lay_cnt, lay_off, steps, grp_off, groups, beta, alpha, p);
+        hard_all + (size_t)cw * (p.K >> 3), circ, lay_cnt, lay_off, steps, grp_off, step_defer, groups, beta,
+        alpha, p);
     if (threadIdx.x == 0) result[cw] = res;
 }
 
@@ -441,7 +492,7 @@ hipError_t launch_ldpc_split_variant(const ldpc_args& a, int batch, hipStream_t 
     hipLaunchKernelGGL(kern, dim3(batch), dim3(LANES * ldpc_dev::SPLIT_LANE_THREADS), lds, stream, a.llr,
                        reinterpret_cast<uint4*>(a.msg), a.hard, a.result,
                        reinterpret_cast<const ldpc_dev::circ_rec*>(a.circ), a.lay_cnt, a.lay_off, a.steps, a.grp_off,
-                       a.groups, a.beta, a.alpha, a.active, p);
+                       a.step_defer, a.groups, a.beta, a.alpha, a.active, p);
     return hipGetLastError();
 }
 

@@ -53,6 +53,7 @@ struct code_desc {
     const int* lay_off;
     const int* steps;
     const int* grp_off;
+    const int* step_defer;
     const int* groups;
     const int* beta;  // learned parameters, or null
     const int* alpha;
@@ -106,7 +107,7 @@ __device__ __forceinline__ int decode_variant(const code_desc& c, unsigned char*
 #define X(ID, C, E)                                                                                         \
     case ID:                                                                                                \
         return ldpc_dev::decode_frame_split<C, E, 2>(lds, llr, rec, h, c.circ, c.lay_cnt, c.lay_off, c.steps, \
-                                                     c.grp_off, c.groups, beta, alpha, p);
+                                                     c.grp_off, c.step_defer, c.groups, beta, alpha, p);
             CUSAT_STREAM_SPLIT_VARIANTS(X)
 #undef X
         }
@@ -333,6 +334,7 @@ int upload_codes(cusat_stream_ctx* sc)
         o.lay_off = c->d_lay_off;
         o.steps = c->d_steps;
         o.grp_off = c->d_grp_off;
+        o.step_defer = c->d_step_defer;
         o.groups = c->d_groups;
         o.beta = c->n_param_iters > 0 ? c->d_beta : nullptr;
         o.alpha = c->n_param_iters > 0 ? c->d_alpha : nullptr;
