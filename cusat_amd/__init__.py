@@ -95,6 +95,9 @@ def lib():
         L.cusat_stream_create.argtypes = [ctypes.POINTER(vp), vp, i, i, u]
         L.cusat_stream_destroy.argtypes = [vp]
         L.cusat_stream_plan.argtypes = [vp, vp, i, vp]
+        L.cusat_stream_plan_async.argtypes = [vp, vp, i, vp]
+        L.cusat_stream_rx_plan_async.argtypes = [vp, vp, i, vp]
+        L.cusat_stream_rx.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.cusat_stream_get_tuning.argtypes = [vp, ctypes.POINTER(StreamTuning), ctypes.POINTER(i)]
         L.cusat_stream_decode.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.cusat_stream_autotune.argtypes = [vp, vp, vp, vp, vp, vp, i, vp, ctypes.POINTER(StreamTuning),
@@ -217,6 +220,9 @@ STREAM_LEARNED = 0x1
 STREAM_ORDER_GIVEN, STREAM_ORDER_COST = 0, 1
 STREAM_FRAME = np.dtype([("code", np.int32), ("reserved", np.int32), ("llr_offset", np.int64),
                          ("hard_offset", np.int64)])
+# cusat_stream_rx_frame: a frame of a mixed receive batch
+STREAM_RX_FRAME = np.dtype([("code", np.int32), ("reserved", np.int32), ("sym_offset", np.int64),
+                            ("out_offset", np.int64), ("noise_var", np.float32), ("llr_gain", np.float32)])
 
 
 class StreamTuning(ctypes.Structure):
@@ -255,6 +261,23 @@ class StreamContext:
                              tuning.get("split_edges", 0))
         _check(lib().cusat_stream_plan(self._h, _ptr(frames), len(frames),
                                        ctypes.byref(t) if t is not None else None), "cusat_stream_plan")
+
+    def plan_async(self, frames, stream=None):
+        """New frames under the current tuning, copied on the stream (no host wait)."""
+        frames = np.ascontiguousarray(frames, STREAM_FRAME)
+        _check(lib().cusat_stream_plan_async(self._h, _ptr(frames), len(frames), _stream(stream)),
+               "cusat_stream_plan_async")
+
+    def rx_plan_async(self, frames, stream=None):
+        """Make a mixed receive batch (STREAM_RX_FRAME records) current."""
+        frames = np.ascontiguousarray(frames, STREAM_RX_FRAME)
+        _check(lib().cusat_stream_rx_plan_async(self._h, _ptr(frames), len(frames), _stream(stream)),
+               "cusat_stream_rx_plan_async")
+
+    def rx(self, sym, out, trials, valid, dfl, stream=None):
+        """Demap, decode, BCH-check and BB-extract the current receive batch."""
+        _check(lib().cusat_stream_rx(self._h, _ptr(sym), _ptr(out), _ptr(trials), _ptr(valid), _ptr(dfl),
+                                     _stream(stream)), "cusat_stream_rx")
 
     def tuning(self):
         t, m = StreamTuning(), ctypes.c_int()

@@ -74,7 +74,7 @@ def compile_one(src):
 
 
 HOST = os.path.join(PKG, "host")
-HOST_SOURCES = ["ldpc_decode.cc", "rx_fec_chain.cc", "rx_fec_pool.cc"]
+HOST_SOURCES = ["ldpc_decode.cc", "rx_fec_chain.cc", "rx_fec_pool.cc", "rx_fec_stream_pool.cc"]
 HOST_LIB = os.path.join(PKG, "libcusat_dsp.so")
 CPP_TESTS = os.path.join(ROOT, "tests", "cpp")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
@@ -102,6 +102,13 @@ def build_host(verbose=True):
                                                  "-lamdhip64", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(ROCM, "lib")],
          "link libcusat_dsp.so")
     os.replace(tmp, HOST_LIB)
+    # the pool throughput benchmark (cusat_amd/pool_bench, next to the libraries)
+    bench = os.path.join(PKG, "pool_bench")
+    _run(["g++"] + flags + [os.path.join(HOST, "pool_bench.cc"), "-o", bench + ".tmp", "-L", PKG,
+                            "-l:libcusat_dsp.so", "-l:libcusat_fec.so", "-L", os.path.join(ROCM, "lib"), "-lamdhip64",
+                            "-lpthread", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(ROCM, "lib")],
+         "compile pool_bench.cc")
+    os.replace(bench + ".tmp", bench)
     out = os.path.join(CPP_TESTS, "_build")
     os.makedirs(out, exist_ok=True)
     for fn in sorted(os.listdir(CPP_TESTS)):

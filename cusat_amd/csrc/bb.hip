@@ -24,6 +24,7 @@
 #include "bb.h"
 #include "ctx.h"
 #include "cusat_fec.h"
+#include "kernels.h"
 
 namespace cusat {
 
@@ -56,6 +57,29 @@ __host__ __device__ uint8_t bb_crc8(const uint8_t* d, int n)
 
 namespace {
 
+// One frame by one 256-thread workgroup: f = its hard bytes, o = its
+// data-field slot (kbch_bytes - 10 bytes); the result goes to *dfl_out.
+__device__ __forceinline__ void bb_frame(const uint8_t* f, int32_t trials, int32_t valid, const uint8_t* prbs,
+                                         int kbch_bytes, uint8_t* o, int32_t* dfl_out, int* n_s)
+{
+    const int max_dfl = kbch_bytes - 10;
+    if (threadIdx.x == 0) {
+        int n = -1;
+        if (trials >= 0 && valid == 1) {
+            uint8_t h[10];
+            for (int i = 0; i < 10; i++) h[i] = f[i] ^ prbs[i];
+            const int dfl = (h[4] << 8) | h[5];
+            n = 0;
+            if (bb_crc8(h, 10) == 0 && (h[0] >> 6) == 0 && dfl % 8 == 0 && dfl <= 8 * max_dfl) n = dfl / 8;
+        }
+        *n_s = n;
+        *dfl_out = n;
+    }
+    __syncthreads();
+    const int n = *n_s;
+    for (int i = threadIdx.x; i < n; i += 256) o[i] = f[10 + i] ^ prbs[10 + i];
+}
+
 __global__ void __launch_bounds__(256)
     bb_extract_kernel(const uint8_t* __restrict__ hard, int hard_stride, const int32_t* __restrict__ trials,
                       const int32_t* __restrict__ valid, const uint8_t* __restrict__ prbs, int kbch_bytes,
@@ -63,24 +87,26 @@ __global__ void __launch_bounds__(256)
 {
     __shared__ int n_s;
     const int b = blockIdx.x;
-    const uint8_t* f = hard + (size_t)b * hard_stride;
-    const int max_dfl = kbch_bytes - 10;
-    if (threadIdx.x == 0) {
-        int n = -1;
-        if (trials[b] >= 0 && valid[b] == 1) {
-            uint8_t h[10];
-            for (int i = 0; i < 10; i++) h[i] = f[i] ^ prbs[i];
-            const int dfl = (h[4] << 8) | h[5];
-            n = 0;
-            if (bb_crc8(h, 10) == 0 && (h[0] >> 6) == 0 && dfl % 8 == 0 && dfl <= 8 * max_dfl) n = dfl / 8;
-        }
-        n_s = n;
-        dfl_out[b] = n;
+    bb_frame(hard + (size_t)b * hard_stride, trials[b], valid[b], prbs, kbch_bytes,
+             out + (size_t)b * (kbch_bytes - 10), dfl_out + b, &n_s);
+}
+
+// Mixed-MODCOD batch: a fixed grid walks the hdr[0] frames (read from
+// device memory, so a captured launch serves any batch).
+__global__ void __launch_bounds__(256)
+    bb_extract_mixed_kernel(const rx_code* __restrict__ codes, const rx_rec* __restrict__ recs,
+                            const int* __restrict__ hdr, const uint8_t* __restrict__ hard,
+                            const int32_t* __restrict__ trials, const int32_t* __restrict__ valid,
+                            uint8_t* __restrict__ out, int32_t* __restrict__ dfl_out)
+{
+    __shared__ int n_s;
+    const int n = hdr[0];
+    for (int b = blockIdx.x; b < n; b += gridDim.x) {
+        const rx_rec& r = recs[b];
+        const rx_code& c = codes[r.code];
+        bb_frame(hard + r.hard_off, trials[b], valid[b], c.prbs, c.kbch_bytes, out + r.out_off, dfl_out + b, &n_s);
+        __syncthreads();  // n_s is rewritten by the next frame
     }
-    __syncthreads();
-    const int n = n_s;
-    uint8_t* o = out + (size_t)b * max_dfl;
-    for (int i = threadIdx.x; i < n; i += 256) o[i] = f[10 + i] ^ prbs[10 + i];
 }
 
 }  // namespace
@@ -91,6 +117,15 @@ hipError_t launch_bb_extract(const uint8_t* hard, int hard_stride, const int32_t
 {
     hipLaunchKernelGGL(bb_extract_kernel, dim3(batch), dim3(256), 0, stream, hard, hard_stride, trials, valid,
                        prbs, kbch_bytes, out, dfl_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_bb_extract_mixed(const rx_code* codes, const rx_rec* recs, const int* hdr, const uint8_t* hard,
+                                   const int32_t* trials, const int32_t* valid, uint8_t* out, int32_t* dfl_out,
+                                   int grid, hipStream_t stream)
+{
+    hipLaunchKernelGGL(bb_extract_mixed_kernel, dim3(grid), dim3(256), 0, stream, codes, recs, hdr, hard, trials,
+                       valid, out, dfl_out);
     return hipGetLastError();
 }
 

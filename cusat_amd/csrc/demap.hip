@@ -64,19 +64,13 @@ __device__ __forceinline__ int16_t to_llr(float v)
     return (int16_t)v;
 }
 
-__global__ void __launch_bounds__(256) demap_kernel(demap_args a)
+// LLRs of symbol j of one frame (sym = the frame's symbols, llr = its N
+// LLRs); pts = the constellation in the LDS (APSK).
+__device__ __forceinline__ void demap_symbol(const demap_args& a, const float* pts, int j, const float2* sym,
+                                             int16_t* llr)
 {
-    __shared__ float pts[512];
-    if (a.pts)
-        for (int i = threadIdx.x; i < 2 * a.n_points; i += blockDim.x) pts[i] = a.pts[i];
-    __syncthreads();
-
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int cw = blockIdx.y;
-    if (j >= a.dlen) return;
-    const float2 y = reinterpret_cast<const float2*>(a.sym)[(size_t)cw * a.xfec + j];
+    const float2 y = sym[j];
     const float re = y.x, im = y.y;
-    int16_t* llr = a.llr + (size_t)cw * a.N;
 
     if (a.n_mod == 2) {
         if (2 * j + 1 < a.N) {
@@ -141,12 +135,76 @@ __global__ void __launch_bounds__(256) demap_kernel(demap_args a)
     }
 }
 
+__global__ void __launch_bounds__(256) demap_kernel(demap_args a)
+{
+    __shared__ float pts[512];
+    if (a.pts)
+        for (int i = threadIdx.x; i < 2 * a.n_points; i += blockDim.x) pts[i] = a.pts[i];
+    __syncthreads();
+
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int cw = blockIdx.y;
+    if (j >= a.dlen) return;
+    demap_symbol(a, pts, j, reinterpret_cast<const float2*>(a.sym) + (size_t)cw * a.xfec, a.llr + (size_t)cw * a.N);
+}
+
+// Mixed-MODCOD batch: a fixed grid walks the batch's 256-symbol blocks
+// (frame f owns blocks recs[f].blk0 .. recs[f + 1].blk0 - 1; hdr[0] =
+// frames, hdr[1] = blocks, both read from device memory so a captured launch
+// serves any batch).  Every symbol gets exactly the arithmetic of demap_kernel.
+__global__ void __launch_bounds__(256)
+    demap_mixed_kernel(const rx_code* __restrict__ codes, const rx_rec* __restrict__ recs,
+                       const int* __restrict__ hdr, const float* __restrict__ sym, int16_t* __restrict__ llr)
+{
+    __shared__ float pts[512];
+    const int n = hdr[0], total = hdr[1];
+    int cur = -1;  // code whose constellation the LDS holds
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        int lo = 0, hi = n - 1;  // last frame with blk0 <= item
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (recs[mid].blk0 <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        const rx_rec& r = recs[lo];
+        const rx_code& c = codes[r.code];
+        demap_args a;
+        a.pts = c.pts;
+        a.xfec = c.xfec;
+        a.dlen = c.dlen;
+        a.N = c.N;
+        a.n_mod = c.n_mod;
+        a.n_points = c.n_points;
+#pragma unroll
+        for (int b = 0; b < 8; b++) a.col[b] = c.col[b];
+        a.qpsk_scale = r.qpsk_scale;
+        a.psk8_scale = r.psk8_scale;
+        a.gain = r.gain;
+        if (c.pts && r.code != cur) {
+            __syncthreads();  // every lane is done with the previous table
+            for (int i = threadIdx.x; i < 2 * c.n_points; i += blockDim.x) pts[i] = c.pts[i];
+            __syncthreads();
+            cur = r.code;
+        }
+        const int j = (item - r.blk0) * 256 + threadIdx.x;
+        if (j < a.dlen)
+            demap_symbol(a, pts, j, reinterpret_cast<const float2*>(sym) + r.sym_off, llr + r.llr_off);
+    }
+}
+
 }  // namespace
 
 hipError_t launch_demap(const demap_args& a, int batch, hipStream_t stream)
 {
     dim3 grid((a.dlen + 255) / 256, batch);
     hipLaunchKernelGGL(demap_kernel, grid, dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_demap_mixed(const rx_code* codes, const rx_rec* recs, const int* hdr, const float* sym,
+                              int16_t* llr, int grid, hipStream_t stream)
+{
+    hipLaunchKernelGGL(demap_mixed_kernel, dim3(grid), dim3(256), 0, stream, codes, recs, hdr, sym, llr);
     return hipGetLastError();
 }
 

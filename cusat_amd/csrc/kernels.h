@@ -61,6 +61,31 @@ struct demap_args {
 };
 hipError_t launch_demap(const demap_args& a, int batch, hipStream_t stream);
 
+// Mixed-MODCOD receive batch (cusat_stream_rx): per code and per frame
+// records in device memory.
+struct rx_code {
+    const float* pts;          // constellation (APSK), null for the QPSK / 8PSK formulas
+    const uint8_t* prbs;       // BB scrambler sequence, kbch_bytes
+    int xfec, dlen, N, n_mod, n_points;
+    int kbch_bytes, hard_bytes;  // k_bch / 8, k_ldpc / 8
+    int pad;
+    uint32_t col[8];
+};
+struct rx_rec {
+    int code, blk0;            // blk0: the frame's first 256-symbol demap block in the batch
+    int64_t sym_off;           // complex symbols
+    int64_t llr_off;           // LLRs
+    int64_t hard_off;          // bytes
+    int64_t out_off;           // data-field bytes
+    float qpsk_scale, psk8_scale, gain, pad2;
+};
+// hdr[0] = frames, hdr[1] = demap blocks of the batch (device memory)
+hipError_t launch_demap_mixed(const rx_code* codes, const rx_rec* recs, const int* hdr, const float* sym,
+                              int16_t* llr, int grid, hipStream_t stream);
+hipError_t launch_bb_extract_mixed(const rx_code* codes, const rx_rec* recs, const int* hdr, const uint8_t* hard,
+                                   const int32_t* trials, const int32_t* valid, uint8_t* out, int32_t* dfl_out,
+                                   int grid, hipStream_t stream);
+
 struct bch_args {
     const uint8_t* hard;       // [B][n_bytes]
     int32_t* valid;            // [B]

@@ -200,7 +200,7 @@ __device__ __forceinline__ int bch_frame(unsigned char* lds, int* flag, const ui
 template <bool WIDE>
 __global__ void __launch_bounds__(stream_threads<WIDE>())
     ldpc_stream_kernel(const code_desc* __restrict__ codes, const cusat_stream_frame* __restrict__ frames,
-                       const int2* __restrict__ jobs, const int* __restrict__ order, int n_jobs,
+                       const int2* __restrict__ jobs, const int* __restrict__ order,
                        unsigned* counters, const int16_t* __restrict__ llr, uint8_t* __restrict__ hard,
                        int32_t* __restrict__ trials, int32_t* __restrict__ valid,
                        int32_t* __restrict__ fallback, uint4* __restrict__ ws, size_t ws_v4, int learned,
@@ -215,7 +215,7 @@ __global__ void __launch_bounds__(stream_threads<WIDE>())
         __syncthreads();
         const int jb = __builtin_amdgcn_readfirstlane(job_s);
         __syncthreads();
-        if (jb >= n_jobs) break;
+        if (jb >= (int)counters[2]) break;  // the plan's job count (device: graph replays see new plans)
         const int2 J = jobs[jb];
         for (int k = 0; k < J.y; k++) {
             const int f = __builtin_amdgcn_readfirstlane(order[J.x + k]);
@@ -281,6 +281,25 @@ struct cusat_stream_ctx {
     int* d_order = nullptr;
     unsigned* d_counters = nullptr;
     uint4* d_ws = nullptr;
+    // plan region (one allocation, this layout on both sides): 256 bytes of
+    // counters ([2] = job count), frames, jobs, order
+    size_t off_jobs = 0, off_order = 0, plan_bytes = 0;
+    static constexpr int N_STAGE = 4;
+    unsigned char* stage[N_STAGE] = {};  // pinned images for cusat_stream_plan_async
+    hipEvent_t stage_done[N_STAGE] = {};
+    bool stage_used[N_STAGE] = {};
+    int next_stage = 0;
+    bool codes_uploaded = false;
+    // receive batches (cusat_stream_rx): per-code records after d_codes, the
+    // rx header (256 bytes: frames, demap blocks) and records at off_rx in the
+    // plan region, LLR and hard-decision work buffers (allocated by the first
+    // rx plan)
+    rx_code* d_rx_codes = nullptr;
+    size_t off_rx = 0;
+    int64_t llr_max = 0, hard_max = 0;  // per frame
+    int16_t* d_llr_work = nullptr;
+    uint8_t* d_hard_work = nullptr;
+    bool rx_planned = false;  // the current plan is an rx plan
     // plan
     int n_frames = 0, n_jobs = 0;
     std::vector<cusat_stream_frame> frames;
@@ -295,29 +314,58 @@ int frame_cost(const cusat_fec_ctx* c)
     return c->kern_steps * (c->cmax + 2);
 }
 
-int build_plan(cusat_stream_ctx* sc, const cusat_stream_tuning& tu)
+// Host image of the plan region for the frames sc->frames under tuning tu:
+// counters (queue and exit count 0, [2] the job count), the frames, the jobs
+// (first position in order, frame count) and the decode order.  Returns the
+// job count.
+int stage_plan(const cusat_stream_ctx* sc, const cusat_stream_tuning& tu, unsigned char* img)
 {
     const int n = sc->n_frames;
-    std::vector<int> order(n);
-    std::iota(order.begin(), order.end(), 0);
+    int* order = reinterpret_cast<int*>(img + sc->off_order);
+    int2* jobs = reinterpret_cast<int2*>(img + sc->off_jobs);
+    std::iota(order, order + n, 0);
     if (tu.order == CUSAT_STREAM_ORDER_COST)
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        std::stable_sort(order, order + n, [&](int a, int b) {
             return frame_cost(sc->codes[sc->frames[a].code]) > frame_cost(sc->codes[sc->frames[b].code]);
         });
-    std::vector<int2> jobs;
+    int nj = 0;
     for (int k = 0; k < n;) {
         int m = 1;
         while (m < tu.frames_per_job && k + m < n &&
                sc->frames[order[k + m]].code == sc->frames[order[k]].code)
             m++;
-        jobs.push_back(make_int2(k, m));
+        jobs[nj++] = make_int2(k, m);
         k += m;
     }
-    sc->n_jobs = (int)jobs.size();
-    if (n) {
-        HIP_TRY(hipMemcpy(sc->d_order, order.data(), n * sizeof(int), hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(sc->d_jobs, jobs.data(), jobs.size() * sizeof(int2), hipMemcpyHostToDevice));
+    unsigned* ctr = reinterpret_cast<unsigned*>(img);
+    memset(ctr, 0, 256);
+    ctr[2] = (unsigned)nj;
+    if (n) memcpy(img + 256, sc->frames.data(), n * sizeof(cusat_stream_frame));
+    return nj;
+}
+
+// Copy the used parts of a plan image (n frames, nj jobs) to the device.
+hipError_t copy_plan(const cusat_stream_ctx* sc, const unsigned char* img, int n, int nj, hipStream_t st, bool async)
+{
+    unsigned char* d = reinterpret_cast<unsigned char*>(sc->d_counters);
+    const size_t part[3][2] = {{0, 256 + n * sizeof(cusat_stream_frame)},
+                               {sc->off_jobs, nj * sizeof(int2)},
+                               {sc->off_order, n * sizeof(int)}};
+    for (const auto& q : part) {
+        if (!q[1]) continue;
+        const hipError_t e = async ? hipMemcpyAsync(d + q[0], img + q[0], q[1], hipMemcpyHostToDevice, st)
+                                   : hipMemcpy(d + q[0], img + q[0], q[1], hipMemcpyHostToDevice);
+        if (e != hipSuccess) return e;
     }
+    return hipSuccess;
+}
+
+int build_plan(cusat_stream_ctx* sc, const cusat_stream_tuning& tu)
+{
+    std::vector<unsigned char> img(sc->plan_bytes);
+    const int nj = stage_plan(sc, tu, img.data());
+    HIP_TRY(copy_plan(sc, img.data(), sc->n_frames, nj, nullptr, false));
+    sc->n_jobs = nj;
     sc->tuning = tu;
     return CUSAT_OK;
 }
@@ -358,6 +406,24 @@ int upload_codes(cusat_stream_ctx* sc)
         o.bch_words = (c->bch.degree + 63) / 64;
     }
     HIP_TRY(hipMemcpy(sc->d_codes, d.data(), d.size() * sizeof(code_desc), hipMemcpyHostToDevice));
+    std::vector<rx_code> r(sc->codes.size());
+    for (size_t k = 0; k < sc->codes.size(); k++) {
+        const cusat_fec_ctx* c = sc->codes[k];
+        rx_code& o = r[k];
+        memset(&o, 0, sizeof o);
+        const bool formula = c->modulation == CUSAT_MOD_QPSK || c->modulation == CUSAT_MOD_8PSK;
+        o.pts = formula ? nullptr : c->d_pts;
+        o.prbs = c->d_prbs;
+        o.xfec = c->xfec;
+        o.dlen = c->dlen;
+        o.N = c->geo.N;
+        o.n_mod = c->con.bits_per_sym;
+        o.n_points = c->con.n_points;
+        o.kbch_bytes = c->fp.k_bch / 8;
+        o.hard_bytes = c->geo.K / 8;
+        for (int b = 0; b < 8; b++) o.col[b] = c->il.col[b];
+    }
+    HIP_TRY(hipMemcpy(sc->d_rx_codes, r.data(), r.size() * sizeof(rx_code), hipMemcpyHostToDevice));
     return CUSAT_OK;
 }
 
@@ -455,8 +521,12 @@ extern "C" int cusat_stream_create(cusat_stream_ctx** out, cusat_fec_ctx* const*
     }
     sc->ws_v4 = rec_max / 16;
     const size_t n_wg = (size_t)sc->n_cu * std::max(sc->max_wg[0], sc->max_wg[1]);
-    size_t bytes = sizeof(code_desc) * n_codes + sizeof(cusat_stream_frame) * max_frames +
-                   sizeof(int2) * max_frames + sizeof(int) * max_frames + 256 + 4096;
+    auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
+    sc->off_jobs = 256 + al(sizeof(cusat_stream_frame) * max_frames);
+    sc->off_order = sc->off_jobs + al(sizeof(int2) * max_frames);
+    sc->off_rx = sc->off_order + al(sizeof(int) * max_frames);
+    sc->plan_bytes = sc->off_rx + 256 + al(sizeof(rx_rec) * max_frames);
+    const size_t bytes = sc->plan_bytes + al(sizeof(code_desc) * n_codes) + al(sizeof(rx_code) * n_codes);
     char* base = nullptr;
     e = hipMalloc(&base, bytes);
     if (e == hipSuccess) e = hipMalloc(&sc->d_ws, n_wg * rec_max);
@@ -466,16 +536,16 @@ extern "C" int cusat_stream_create(cusat_stream_ctx** out, cusat_fec_ctx* const*
         set_last_error("stream buffers: %s", hipGetErrorString(e));
         return CUSAT_E_NOMEM;
     }
-    auto take = [&](size_t n) {
-        char* p = base;
-        base += (n + 255) & ~(size_t)255;
-        return p;
-    };
-    sc->d_counters = reinterpret_cast<unsigned*>(take(256));
-    sc->d_codes = reinterpret_cast<code_desc*>(take(sizeof(code_desc) * n_codes));
-    sc->d_frames = reinterpret_cast<cusat_stream_frame*>(take(sizeof(cusat_stream_frame) * max_frames));
-    sc->d_jobs = reinterpret_cast<int2*>(take(sizeof(int2) * max_frames));
-    sc->d_order = reinterpret_cast<int*>(take(sizeof(int) * max_frames));
+    sc->d_counters = reinterpret_cast<unsigned*>(base);
+    sc->d_frames = reinterpret_cast<cusat_stream_frame*>(base + 256);
+    sc->d_jobs = reinterpret_cast<int2*>(base + sc->off_jobs);
+    sc->d_order = reinterpret_cast<int*>(base + sc->off_order);
+    sc->d_codes = reinterpret_cast<code_desc*>(base + sc->plan_bytes);
+    sc->d_rx_codes = reinterpret_cast<rx_code*>(base + sc->plan_bytes + al(sizeof(code_desc) * n_codes));
+    for (const cusat_fec_ctx* c : sc->codes) {
+        sc->llr_max = std::max<int64_t>(sc->llr_max, c->geo.N);
+        sc->hard_max = std::max<int64_t>(sc->hard_max, c->geo.K / 8);
+    }
     if (hipMemset(sc->d_counters, 0, 256) != hipSuccess) {
         cusat_stream_destroy(sc);
         set_last_error("counter init failed");
@@ -495,8 +565,17 @@ extern "C" int cusat_stream_create(cusat_stream_ctx** out, cusat_fec_ctx* const*
 extern "C" int cusat_stream_destroy(cusat_stream_ctx* sc)
 {
     if (!sc) return CUSAT_OK;
+    for (int k = 0; k < cusat_stream_ctx::N_STAGE; k++) {
+        if (sc->stage_done[k]) {
+            hipEventSynchronize(sc->stage_done[k]);
+            hipEventDestroy(sc->stage_done[k]);
+        }
+        if (sc->stage[k]) hipHostFree(sc->stage[k]);
+    }
     if (sc->d_counters) hipFree(sc->d_counters);
     if (sc->d_ws) hipFree(sc->d_ws);
+    if (sc->d_llr_work) hipFree(sc->d_llr_work);
+    if (sc->d_hard_work) hipFree(sc->d_hard_work);
     delete sc;
     return CUSAT_OK;
 }
@@ -526,9 +605,46 @@ extern "C" int cusat_stream_plan(cusat_stream_ctx* sc, const cusat_stream_frame*
     sc->n_frames = n_frames;
     int rc = upload_codes(sc);
     if (rc != CUSAT_OK) return rc;
-    if (n_frames)
-        HIP_TRY(hipMemcpy(sc->d_frames, frames, n_frames * sizeof(cusat_stream_frame), hipMemcpyHostToDevice));
+    sc->codes_uploaded = true;
+    sc->rx_planned = false;
     return build_plan(sc, tu);
+}
+
+extern "C" int cusat_stream_plan_async(cusat_stream_ctx* sc, const cusat_stream_frame* frames, int n_frames,
+                                       void* stream)
+{
+    if (!sc || (n_frames > 0 && !frames) || n_frames < 0 || n_frames > sc->max_frames) {
+        set_last_error("bad argument (n_frames %d, max %d)", n_frames, sc ? sc->max_frames : 0);
+        return CUSAT_E_INVALID;
+    }
+    for (int k = 0; k < n_frames; k++)
+        if (frames[k].code < 0 || frames[k].code >= (int)sc->codes.size() || frames[k].llr_offset < 0 ||
+            frames[k].hard_offset < 0) {
+            set_last_error("frame %d: code %d / offsets out of range", k, frames[k].code);
+            return CUSAT_E_INVALID;
+        }
+    if (!sc->codes_uploaded) {
+        const int rc = upload_codes(sc);
+        if (rc != CUSAT_OK) return rc;
+        sc->codes_uploaded = true;
+    }
+    const int k = sc->next_stage;
+    if (!sc->stage[k]) {
+        HIP_TRY(hipHostMalloc((void**)&sc->stage[k], sc->plan_bytes, hipHostMallocDefault));
+        HIP_TRY(hipEventCreateWithFlags(&sc->stage_done[k], hipEventDisableTiming));
+    }
+    if (sc->stage_used[k]) HIP_TRY(hipEventSynchronize(sc->stage_done[k]));  // its last copy has run
+    sc->frames.assign(frames, frames + n_frames);
+    sc->n_frames = n_frames;
+    const int nj = stage_plan(sc, sc->tuning, sc->stage[k]);
+    hipStream_t st = (hipStream_t)stream;
+    HIP_TRY(copy_plan(sc, sc->stage[k], n_frames, nj, st, true));
+    HIP_TRY(hipEventRecord(sc->stage_done[k], st));
+    sc->stage_used[k] = true;
+    sc->next_stage = (k + 1) % cusat_stream_ctx::N_STAGE;
+    sc->n_jobs = nj;
+    sc->rx_planned = false;
+    return CUSAT_OK;
 }
 
 extern "C" int cusat_stream_get_tuning(const cusat_stream_ctx* sc, cusat_stream_tuning* out, int* max_wg_per_cu)
@@ -539,25 +655,122 @@ extern "C" int cusat_stream_get_tuning(const cusat_stream_ctx* sc, cusat_stream_
     return CUSAT_OK;
 }
 
-extern "C" int cusat_stream_decode(cusat_stream_ctx* sc, const int16_t* llr, uint8_t* hard, int32_t* trials,
-                                   int32_t* valid, int32_t* fallback, void* stream)
+namespace {
+
+hipError_t launch_stream(cusat_stream_ctx* sc, const int16_t* llr, uint8_t* hard, int32_t* trials, int32_t* valid,
+                         int32_t* fallback, void* stream)
 {
-    if (!sc || !llr || !hard || !trials || !valid) { set_last_error("bad argument"); return CUSAT_E_INVALID; }
-    if (sc->n_jobs == 0) return CUSAT_OK;
     const int grid = sc->n_cu * sc->tuning.wg_per_cu;
     const int w = sc->tuning.split_edges;
     const int learned = (sc->flags & CUSAT_STREAM_LEARNED) ? 1 : 0;
     if (w)
         hipLaunchKernelGGL(ldpc_stream_kernel<true>, dim3(grid), dim3(stream_threads<true>()), sc->lds[1],
-                           (hipStream_t)stream, sc->d_codes, sc->d_frames, sc->d_jobs, sc->d_order, sc->n_jobs,
+                           (hipStream_t)stream, sc->d_codes, sc->d_frames, sc->d_jobs, sc->d_order,
                            sc->d_counters, llr, hard, trials, valid, fallback, sc->d_ws, sc->ws_v4, learned,
                            sc->lds[1] - 16);
     else
         hipLaunchKernelGGL(ldpc_stream_kernel<false>, dim3(grid), dim3(stream_threads<false>()), sc->lds[0],
-                           (hipStream_t)stream, sc->d_codes, sc->d_frames, sc->d_jobs, sc->d_order, sc->n_jobs,
+                           (hipStream_t)stream, sc->d_codes, sc->d_frames, sc->d_jobs, sc->d_order,
                            sc->d_counters, llr, hard, trials, valid, fallback, sc->d_ws, sc->ws_v4, learned,
                            sc->lds[0] - 16);
-    HIP_TRY(hipGetLastError());
+    return hipGetLastError();
+}
+
+}  // namespace
+
+// Launches even for an empty plan (every workgroup leaves at once), so a
+// captured launch exists whatever plan it later replays.
+extern "C" int cusat_stream_decode(cusat_stream_ctx* sc, const int16_t* llr, uint8_t* hard, int32_t* trials,
+                                   int32_t* valid, int32_t* fallback, void* stream)
+{
+    if (!sc || !llr || !hard || !trials || !valid) { set_last_error("bad argument"); return CUSAT_E_INVALID; }
+    HIP_TRY(launch_stream(sc, llr, hard, trials, valid, fallback, stream));
+    return CUSAT_OK;
+}
+
+extern "C" int cusat_stream_rx_plan_async(cusat_stream_ctx* sc, const cusat_stream_rx_frame* frames, int n_frames,
+                                          void* stream)
+{
+    if (!sc || (n_frames > 0 && !frames) || n_frames < 0 || n_frames > sc->max_frames) {
+        set_last_error("bad argument (n_frames %d, max %d)", n_frames, sc ? sc->max_frames : 0);
+        return CUSAT_E_INVALID;
+    }
+    for (int k = 0; k < n_frames; k++)
+        if (frames[k].code < 0 || frames[k].code >= (int)sc->codes.size() || frames[k].sym_offset < 0 ||
+            frames[k].out_offset < 0 || !(frames[k].noise_var > 0.0f)) {
+            set_last_error("rx frame %d: code %d / offsets / noise_var out of range", k, frames[k].code);
+            return CUSAT_E_INVALID;
+        }
+    if (!sc->codes_uploaded) {
+        const int rc = upload_codes(sc);
+        if (rc != CUSAT_OK) return rc;
+        sc->codes_uploaded = true;
+    }
+    if (!sc->d_llr_work) {
+        HIP_TRY(hipMalloc(&sc->d_llr_work, (size_t)sc->max_frames * sc->llr_max * sizeof(int16_t)));
+        HIP_TRY(hipMalloc(&sc->d_hard_work, (size_t)sc->max_frames * sc->hard_max));
+    }
+    const int k = sc->next_stage;
+    if (!sc->stage[k]) {
+        HIP_TRY(hipHostMalloc((void**)&sc->stage[k], sc->plan_bytes, hipHostMallocDefault));
+        HIP_TRY(hipEventCreateWithFlags(&sc->stage_done[k], hipEventDisableTiming));
+    }
+    if (sc->stage_used[k]) HIP_TRY(hipEventSynchronize(sc->stage_done[k]));  // its last copy has run
+    unsigned char* img = sc->stage[k];
+    int* hdr = reinterpret_cast<int*>(img + sc->off_rx);
+    rx_rec* rec = reinterpret_cast<rx_rec*>(img + sc->off_rx + 256);
+    // the decoder's frames: LLRs and hard decisions packed in the work buffers
+    sc->frames.resize(n_frames);
+    int64_t lo = 0, ho = 0;
+    int blk = 0;
+    for (int f = 0; f < n_frames; f++) {
+        const cusat_fec_ctx* c = sc->codes[frames[f].code];
+        sc->frames[f] = cusat_stream_frame{frames[f].code, 0, lo, ho};
+        rx_rec& r = rec[f];
+        memset(&r, 0, sizeof r);
+        r.code = frames[f].code;
+        r.blk0 = blk;
+        r.sym_off = frames[f].sym_offset;
+        r.llr_off = lo;
+        r.hard_off = ho;
+        r.out_off = frames[f].out_offset;
+        cusat_fec_demap_scales(c, frames[f].noise_var, frames[f].llr_gain, &r.qpsk_scale, &r.psk8_scale, &r.gain);
+        lo += c->geo.N;
+        ho += c->geo.K / 8;
+        blk += (c->dlen + 255) / 256;
+    }
+    memset(hdr, 0, 256);
+    hdr[0] = n_frames;
+    hdr[1] = blk;
+    sc->n_frames = n_frames;
+    const int nj = stage_plan(sc, sc->tuning, img);
+    hipStream_t st = (hipStream_t)stream;
+    HIP_TRY(copy_plan(sc, img, n_frames, nj, st, true));
+    HIP_TRY(hipMemcpyAsync(reinterpret_cast<unsigned char*>(sc->d_counters) + sc->off_rx, img + sc->off_rx,
+                           256 + n_frames * sizeof(rx_rec), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipEventRecord(sc->stage_done[k], st));
+    sc->stage_used[k] = true;
+    sc->next_stage = (k + 1) % cusat_stream_ctx::N_STAGE;
+    sc->n_jobs = nj;
+    sc->rx_planned = true;
+    return CUSAT_OK;
+}
+
+extern "C" int cusat_stream_rx(cusat_stream_ctx* sc, const float* sym, uint8_t* out, int32_t* trials,
+                               int32_t* valid, int32_t* dfl, void* stream)
+{
+    if (!sc || !sym || !out || !trials || !valid || !dfl) { set_last_error("bad argument"); return CUSAT_E_INVALID; }
+    if (!sc->rx_planned) {
+        set_last_error("cusat_stream_rx needs an rx plan (cusat_stream_rx_plan_async)");
+        return CUSAT_E_INVALID;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    const int* hdr = reinterpret_cast<const int*>(reinterpret_cast<unsigned char*>(sc->d_counters) + sc->off_rx);
+    const rx_rec* rec = reinterpret_cast<const rx_rec*>(hdr + 64);
+    HIP_TRY(launch_demap_mixed(sc->d_rx_codes, rec, hdr, sym, sc->d_llr_work, sc->n_cu * 4, st));
+    HIP_TRY(launch_stream(sc, sc->d_llr_work, sc->d_hard_work, trials, valid, nullptr, stream));
+    HIP_TRY(launch_bb_extract_mixed(sc->d_rx_codes, rec, hdr, sc->d_hard_work, trials, valid, out, dfl, sc->n_cu * 2,
+                                    st));
     return CUSAT_OK;
 }
 

@@ -1,0 +1,168 @@
+// ACM receive throughput of the two pools of the cusat_dsp API on one GPU:
+// rx_fec_pool (one rx_fec_chain per MODCOD, a flush on every MODCOD change,
+// rx_fec_pool.cc:55-77) and rx_fec_stream_pool (one ring of mixed
+// micro-batches, each a graph replay of cusat_stream_rx).
+//
+// Workload: normal FECFRAMEs of the given MODCODs, each carrying a GSE data
+// field (BBHEADER + payload, BB-scrambled), encoded, mapped and noised by the
+// library's host transmitter at threshold + 1.5 dB; the frames arrive in runs
+// of one MODCOD whose length is uniform in 1..run.  Timed: every submit, the
+// output collection and finish_all_pending, from host symbols to GSE bytes
+// (host copy into pinned slots, H2D, demap, LDPC, BCH, BB, D2H included).
+//
+// usage: pool_bench [frames] [run] [pool: chain|stream|both] [micro_batch] [streams] [modcods,comma-separated]
+// prints one JSON line per pool.
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "cusat_dsp/rx_fec_pool.h"
+#include "cusat_dsp/rx_fec_stream_pool.h"
+#include "cusat_fec.h"
+
+using namespace cusat_dsp;
+
+namespace {
+
+struct source {
+    const modcod_def_t* m = nullptr;
+    int xfec = 0, kbch = 0;
+    std::vector<complexf> symbols;  // frames_per_source x xfec
+};
+
+constexpr int FRAMES_PER_SOURCE = 48;
+
+source make_source(const modcod_def_t* m, uint32_t seed)
+{
+    cusat_fec_info inf;
+    if (cusat_fec_query(FECFRAME_NORMAL, m->code_rate, m->constellation, 0, &inf) != CUSAT_OK) {
+        std::fprintf(stderr, "%s\n", cusat_last_error());
+        std::exit(2);
+    }
+    source s;
+    s.m = m;
+    s.xfec = inf.xfec_len;
+    s.kbch = inf.k_bch / 8;
+    std::vector<uint8_t> prbs(s.kbch);
+    cusat_bb_prbs(prbs.data(), s.kbch);
+    std::mt19937 rng(seed);
+    std::vector<uint8_t> info((size_t)FRAMES_PER_SOURCE * s.kbch);
+    for (int b = 0; b < FRAMES_PER_SOURCE; b++) {
+        uint8_t* f = &info[(size_t)b * s.kbch];
+        for (int i = 0; i < s.kbch; i++) f[i] = (uint8_t)rng();
+        const int len = (int)(rng() % (s.kbch - 10));
+        f[0] = (1 << 5) | (1 << 4) | 2;  // generic packetized, SIS, CCM, roll-off 0.20
+        f[1] = f[2] = f[3] = 0;
+        f[4] = (uint8_t)((8 * len) >> 8);
+        f[5] = (uint8_t)(8 * len);
+        f[6] = 0;
+        f[7] = f[8] = 0xff;
+        f[9] = (uint8_t)cusat_bb_crc8(f, 9);
+        for (int i = 0; i < s.kbch; i++) f[i] ^= prbs[i];
+    }
+    s.symbols.resize((size_t)FRAMES_PER_SOURCE * s.xfec);
+    if (cusat_encode_frames(FECFRAME_NORMAL, m->code_rate, m->constellation, seed, FRAMES_PER_SOURCE,
+                            m->min_snr_db + 1.5f, info.data(), reinterpret_cast<float*>(s.symbols.data()),
+                            nullptr) != CUSAT_OK) {
+        std::fprintf(stderr, "%s\n", cusat_last_error());
+        std::exit(2);
+    }
+    return s;
+}
+
+struct arrival {
+    int src, frame;
+};
+
+template <class P>
+void run(P& pool, const char* name, const std::vector<source>& src, const std::vector<arrival>& arr, int run_len,
+         int micro_batch, int streams)
+{
+    for (const auto& s : src) pool.update_snr(s.m->modcod_idx, s.m->min_snr_db + 1.5f);
+    pool.warmup_all();
+    for (const auto& s : src) pool.update_snr(s.m->modcod_idx, s.m->min_snr_db + 1.5f);
+    std::vector<uint8_t> buf(pool.max_output_bytes());
+    uint64_t out_bytes = 0, info_bits = 0;
+    auto collect = [&] {
+        for (;;) {
+            const int n = pool.get_gse_output(buf.data(), (int)buf.size());
+            if (n <= 0) break;
+            out_bytes += (uint64_t)n;
+        }
+    };
+    pl_layout l;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (const auto& a : arr) {
+        const source& s = src[a.src];
+        l.modcod = s.m->modcod_idx;
+        if (pool.submit(&s.symbols[(size_t)a.frame * s.xfec], s.xfec, l)) collect();
+        info_bits += 8ull * s.kbch;
+    }
+    pool.finish_all_pending();
+    collect();
+    const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::printf("{\"pool\": \"%s\", \"frames\": %zu, \"modcods\": %zu, \"max_run\": %d, \"micro_batch\": %d, "
+                "\"streams\": %d, \"seconds\": %.4f, \"frames_per_s\": %.1f, \"gbps_info\": %.4f, "
+                "\"gse_bytes\": %llu, \"frame_errors\": %llu, \"drops\": %llu}\n",
+                name, arr.size(), src.size(), run_len, micro_batch, streams, sec, arr.size() / sec,
+                info_bits / sec * 1e-9, (unsigned long long)out_bytes,
+                (unsigned long long)pool.total_error_count(), (unsigned long long)pool.total_gpu_drop_count());
+    std::fflush(stdout);
+}
+
+}  // namespace
+
+int main(int argc, char** argv)
+{
+    const int n_frames = argc > 1 ? std::atoi(argv[1]) : 3072;
+    const int run_len = argc > 2 ? std::atoi(argv[2]) : 8;
+    const std::string which = argc > 3 ? argv[3] : "both";
+    const int micro_batch = argc > 4 ? std::atoi(argv[4]) : 128;
+    const int streams = argc > 5 ? std::atoi(argv[5]) : 4;
+    // QPSK 1/4, 1/2, 3/4, 2/3; 8PSK 3/5, 9/10, 3/4; 16APSK 2/3, 5/6, 3/4; 32APSK 3/4, 9/10
+    std::vector<uint8_t> ids = {1, 4, 7, 6, 12, 17, 14, 18, 21, 19, 24, 28};
+    if (argc > 6) {
+        ids.clear();
+        std::stringstream ss(argv[6]);
+        std::string tok;
+        while (std::getline(ss, tok, ',')) ids.push_back((uint8_t)std::atoi(tok.c_str()));
+    }
+    if (n_frames < 1 || run_len < 1 || micro_batch < 1 || streams < 1 || ids.empty()) {
+        std::fprintf(stderr, "bad arguments\n");
+        return 2;
+    }
+    std::vector<source> src;
+    for (size_t k = 0; k < ids.size(); k++) {
+        const modcod_def_t* m = find_modcod(ids[k]);
+        if (!m) {
+            std::fprintf(stderr, "unknown MODCOD %d\n", ids[k]);
+            return 2;
+        }
+        src.push_back(make_source(m, 100 + (uint32_t)k));
+    }
+    std::mt19937 rng(7);
+    std::vector<arrival> arr;
+    int next = 0;
+    while ((int)arr.size() < n_frames) {
+        const int k = (int)(rng() % src.size());
+        const int r = 1 + (int)(rng() % run_len);
+        for (int j = 0; j < r && (int)arr.size() < n_frames; j++) arr.push_back({k, next++ % FRAMES_PER_SOURCE});
+    }
+    std::printf("# %zu frames generated\n", arr.size());
+    std::fflush(stdout);
+    if (which == "chain" || which == "both") {
+        // the reference's defaults: micro-batches of 4 over 8 streams per MODCOD
+        rx_fec_pool pool(FECFRAME_NORMAL, ids, 25, 64, 8, 4);
+        run(pool, "rx_fec_pool", src, arr, run_len, 4, 8);
+    }
+    if (which == "stream" || which == "both") {
+        rx_fec_stream_pool pool(FECFRAME_NORMAL, ids, 25, 64, streams, micro_batch);
+        run(pool, "rx_fec_stream_pool", src, arr, run_len, micro_batch, streams);
+    }
+    return 0;
+}

@@ -240,7 +240,46 @@ int cusat_stream_destroy(cusat_stream_ctx* sc);
  * current one; the autotuner's choice after cusat_stream_autotune). */
 int cusat_stream_plan(cusat_stream_ctx* sc, const cusat_stream_frame* frames, int n_frames,
                       const cusat_stream_tuning* tuning);
+/* cusat_stream_plan with the current tuning, without host synchronisation:
+ * the plan is staged in pinned memory and copied on `stream`, so decodes
+ * enqueued on that stream before this call run the previous plan and those
+ * after it this one.  The launch reads its job count from device memory, so
+ * a captured decode graph replays whatever plan is current when it runs.
+ * For a caller that replans every micro-batch (an ACM receiver); the host
+ * blocks only when four plans are still waiting for their copies. */
+int cusat_stream_plan_async(cusat_stream_ctx* sc, const cusat_stream_frame* frames, int n_frames,
+                            void* stream);
 int cusat_stream_get_tuning(const cusat_stream_ctx* sc, cusat_stream_tuning* out, int* max_wg_per_cu);
+
+/* ---- mixed ACM receive batch: demap -> LDPC -> BCH -> BB in three launches ----
+ * The device sequence of rx_fec_chain::launch_slot_gpu + post_process_slot
+ * (rx_fec_chain.cc:400-481, 565-627) for a micro-batch whose frames belong to
+ * any of the stream context's MODCODs, in arrival order: one demap launch
+ * over every frame (each with its own scaling), the persistent decode launch,
+ * one BB launch.  Per frame the outputs equal cusat_fec_demap +
+ * cusat_ldpc_decode + cusat_bch_check + cusat_bb_extract of the frame's own
+ * context.  The three launches read the batch from device memory, so a
+ * captured cusat_stream_rx graph replays any plan. */
+typedef struct cusat_stream_rx_frame {
+    int32_t code;          /* index into the contexts of cusat_stream_create */
+    int32_t reserved;
+    int64_t sym_offset;    /* first symbol (complex float pair) of the XFECFRAME in sym */
+    int64_t out_offset;    /* first byte of the frame's data-field slot (k_bch/8 - 10 bytes) in out */
+    float noise_var;       /* demapper scaling, as cusat_fec_demap takes it */
+    float llr_gain;
+} cusat_stream_rx_frame;
+/* Make the micro-batch current (current tuning; staged and copied on
+ * `stream` like cusat_stream_plan_async).  LLRs and hard decisions live in
+ * work buffers of the stream context (max_frames x the largest code,
+ * allocated by the first call).  A later cusat_stream_plan /
+ * cusat_stream_plan_async replaces it. */
+int cusat_stream_rx_plan_async(cusat_stream_ctx* sc, const cusat_stream_rx_frame* frames, int n_frames,
+                               void* stream);
+/* Run the current rx plan: sym = device symbols, out = device data-field
+ * slots, trials / valid / dfl = B ints each by frame number (dfl as
+ * cusat_bb_extract writes it: -1 FEC failure, 0 no data field, n bytes). */
+int cusat_stream_rx(cusat_stream_ctx* sc, const float* sym, uint8_t* out, int32_t* trials, int32_t* valid,
+                    int32_t* dfl, void* stream);
 /* Decode the planned frames: llr (device int16), hard (device bytes),
  * trials / valid / fallback (B ints each, fallback may be NULL) indexed by
  * frame number. */

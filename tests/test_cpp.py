@@ -33,3 +33,8 @@ def test_cpp_rx_fec_chain_on_gpu():
 @pytest.mark.gpu
 def test_cpp_rx_fec_pool_on_gpu():
     _run("qa_rx_fec_pool_gpu", 110)
+
+
+@pytest.mark.gpu
+def test_cpp_rx_fec_stream_pool_on_gpu():
+    _run("qa_rx_fec_stream_pool_gpu", 110)

@@ -179,3 +179,95 @@ def test_stream_graph_replays():
         got = (hard.cpu().numpy(), tr.cpu().numpy(), va.cpu().numpy(), np.zeros(n, np.int32))
         ref = per_code_reference(ctxs, by_new)
         check(ctxs, frames_new, who, ref, got)
+
+
+def _subset(frames, who, pick, ctxs):
+    """Frame records of the frames `pick` (indices into frames), packed into
+    fresh llr / hard positions in the given order, and the slices of the
+    original llr buffer to gather."""
+    sub = np.zeros(len(pick), ca.STREAM_FRAME)
+    lo = ho = 0
+    src = []
+    for n, p in enumerate(pick):
+        k = who[p][0]
+        sub[n] = (k, 0, lo, ho)
+        src.append((int(frames[p]["llr_offset"]), ctxs[k].info["n_ldpc"]))
+        lo += ctxs[k].info["n_ldpc"]
+        ho += ctxs[k].info["k_ldpc"] // 8
+    return sub, src, lo, ho
+
+
+def test_stream_graph_replays_new_plans():
+    """A captured launch replays whatever plan cusat_stream_plan_async made
+    current: fewer frames, other codes, another order, an empty plan."""
+    torch = _torch()
+    mix = MIX[:3] + MIX[5:7]
+    ctxs, frames, llr, by_code, who, hb = build_stream(mix, 4, 12)
+    ref = per_code_reference(ctxs, by_code)
+    n = len(frames)
+    sc = ca.StreamContext(ctxs, n)
+    sc.plan(frames)
+    llr_w = torch.zeros_like(llr)
+    hard = torch.zeros(hb, dtype=torch.uint8, device="cuda")
+    tr = torch.zeros(n, dtype=torch.int32, device="cuda")
+    va = torch.zeros(n, dtype=torch.int32, device="cuda")
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        sc.decode(llr_w, hard, tr, va)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        sc.decode(llr_w, hard, tr, va)
+    rng = np.random.default_rng(5)
+    picks = [rng.permutation(n)[:7], np.arange(n)[::-1], np.array([], np.int64), rng.permutation(n)]
+    for pick in picks:
+        sub, src, lo, ho = _subset(frames, who, pick, ctxs)
+        if len(pick):
+            llr_w[:lo].copy_(torch.cat([llr[o:o + ln] for o, ln in src]))
+        tr.fill_(-7)
+        va.fill_(-7)
+        hard.zero_()
+        sc.plan_async(sub)  # on the current stream, before the replay
+        g.replay()
+        torch.cuda.synchronize()
+        t, v, h = tr.cpu().numpy(), va.cpu().numpy(), hard.cpu().numpy()
+        for m, p in enumerate(pick):
+            k, j = who[p]
+            kb = ctxs[k].info["k_ldpc"] // 8
+            assert t[m] == ref[k][1][j] and v[m] == ref[k][2][j], (m, k, j)
+            np.testing.assert_array_equal(h[sub[m]["hard_offset"]:sub[m]["hard_offset"] + kb], ref[k][0][j])
+        assert (t[len(pick):] == -7).all()  # frames outside the plan untouched
+
+
+def test_stream_plan_async_back_to_back():
+    """More plans in flight on one stream than the pinned staging ring
+    holds, with no host synchronisation between plan and decode: every
+    decode runs the plan made just before it."""
+    torch = _torch()
+    mix = MIX[:2] + MIX[5:6]
+    ctxs, frames, llr, by_code, who, hb = build_stream(mix, 3, 13)
+    ref = per_code_reference(ctxs, by_code)
+    n = len(frames)
+    sc = ca.StreamContext(ctxs, n)
+    rng = np.random.default_rng(9)
+    outs = []
+    for r in range(9):
+        pick = rng.permutation(n)[: 1 + r % n]
+        sub, src, lo, ho = _subset(frames, who, pick, ctxs)
+        lw = torch.cat([llr[o:o + ln] for o, ln in src])
+        h = torch.zeros(ho, dtype=torch.uint8, device="cuda")
+        t = torch.full((len(pick),), -7, dtype=torch.int32, device="cuda")
+        v = torch.full((len(pick),), -7, dtype=torch.int32, device="cuda")
+        sc.plan_async(sub)
+        sc.decode(lw, h, t, v)
+        outs.append((pick, sub, h, t, v, lw))
+    torch.cuda.synchronize()
+    for pick, sub, h, t, v, _ in outs:
+        t, v, h = t.cpu().numpy(), v.cpu().numpy(), h.cpu().numpy()
+        for m, p in enumerate(pick):
+            k, j = who[p]
+            kb = ctxs[k].info["k_ldpc"] // 8
+            assert t[m] == ref[k][1][j] and v[m] == ref[k][2][j], (m, k, j)
+            np.testing.assert_array_equal(h[sub[m]["hard_offset"]:sub[m]["hard_offset"] + kb], ref[k][0][j])
