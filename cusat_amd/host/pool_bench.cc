@@ -21,6 +21,8 @@
 #include <string>
 #include <vector>
 
+#include <hip/hip_runtime_api.h>
+
 #include "cusat_dsp/rx_fec_pool.h"
 #include "cusat_dsp/rx_fec_stream_pool.h"
 #include "cusat_fec.h"
@@ -96,20 +98,24 @@ void run(P& pool, const char* name, const std::vector<source>& src, const std::v
         }
     };
     pl_layout l;
+    double t_submit = 0;
     const auto t0 = std::chrono::steady_clock::now();
     for (const auto& a : arr) {
         const source& s = src[a.src];
         l.modcod = s.m->modcod_idx;
-        if (pool.submit(&s.symbols[(size_t)a.frame * s.xfec], s.xfec, l)) collect();
+        const auto ts = std::chrono::steady_clock::now();
+        const bool ready = pool.submit(&s.symbols[(size_t)a.frame * s.xfec], s.xfec, l);
+        t_submit += std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
+        if (ready) collect();
         info_bits += 8ull * s.kbch;
     }
     pool.finish_all_pending();
     collect();
     const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     std::printf("{\"pool\": \"%s\", \"frames\": %zu, \"modcods\": %zu, \"max_run\": %d, \"micro_batch\": %d, "
-                "\"streams\": %d, \"seconds\": %.4f, \"frames_per_s\": %.1f, \"gbps_info\": %.4f, "
-                "\"gse_bytes\": %llu, \"frame_errors\": %llu, \"drops\": %llu}\n",
-                name, arr.size(), src.size(), run_len, micro_batch, streams, sec, arr.size() / sec,
+                "\"streams\": %d, \"seconds\": %.4f, \"seconds_in_submit\": %.4f, \"frames_per_s\": %.1f, "
+                "\"gbps_info\": %.4f, \"gse_bytes\": %llu, \"frame_errors\": %llu, \"drops\": %llu}\n",
+                name, arr.size(), src.size(), run_len, micro_batch, streams, sec, t_submit, arr.size() / sec,
                 info_bits / sec * 1e-9, (unsigned long long)out_bytes,
                 (unsigned long long)pool.total_error_count(), (unsigned long long)pool.total_gpu_drop_count());
     std::fflush(stdout);
@@ -155,6 +161,73 @@ int main(int argc, char** argv)
     }
     std::printf("# %zu frames generated\n", arr.size());
     std::fflush(stdout);
+    {
+        // floor of any pool taking host symbols: one memcpy of every frame
+        // into pinned memory (a ring of 128 frame slots), on this thread
+        size_t max_xfec = 0, bytes = 0;
+        for (const auto& s : src) max_xfec = std::max<size_t>(max_xfec, s.xfec);
+        complexf* pinned = nullptr;
+        if (hipHostMalloc((void**)&pinned, 128 * max_xfec * sizeof(complexf), hipHostMallocDefault) != hipSuccess) {
+            std::fprintf(stderr, "hipHostMalloc failed\n");
+            return 2;
+        }
+        for (int pass = 0; pass < 2; pass++) {  // the first pass touches the pages
+            const auto t0 = std::chrono::steady_clock::now();
+            size_t k = 0;
+            bytes = 0;
+            for (const auto& a : arr) {
+                const source& s = src[a.src];
+                memcpy(pinned + (k++ % 128) * max_xfec, &s.symbols[(size_t)a.frame * s.xfec], s.xfec * sizeof(complexf));
+                bytes += s.xfec * sizeof(complexf);
+            }
+            const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (pass == 1)
+                std::printf("{\"host_copy_floor\": true, \"frames\": %zu, \"bytes\": %zu, \"seconds\": %.4f, "
+                            "\"frames_per_s\": %.1f, \"gb_per_s\": %.2f}\n",
+                            arr.size(), bytes, sec, arr.size() / sec, bytes / sec * 1e-9);
+        }
+        hipHostFree(pinned);
+        std::fflush(stdout);
+    }
+    {
+        // host -> device copy rate of micro-batch sized blocks (32 MiB) from
+        // pinned memory of each allocation kind, on one stream
+        const size_t blk = 32u << 20;
+        const struct {
+            unsigned flags;
+            const char* name;
+        } kinds[] = {{hipHostMallocDefault, "default"},
+                     {hipHostMallocNonCoherent, "non_coherent"},
+                     {hipHostMallocCoherent, "coherent"}};
+        void* dev = nullptr;
+        hipStream_t st = nullptr;
+        if (hipMalloc(&dev, blk) != hipSuccess || hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+            std::fprintf(stderr, "device buffer failed\n");
+            return 2;
+        }
+        for (const auto& k : kinds) {
+            void* h = nullptr;
+            if (hipHostMalloc(&h, blk, k.flags) != hipSuccess) continue;
+            memset(h, 1, blk);
+            double host_s = 0, total_s = 0;
+            for (int pass = 0; pass < 2; pass++) {
+                const auto t0 = std::chrono::steady_clock::now();
+                for (int r = 0; r < 16; r++) hipMemcpyAsync(dev, h, blk, hipMemcpyHostToDevice, st);
+                const auto t1 = std::chrono::steady_clock::now();
+                hipStreamSynchronize(st);
+                const auto t2 = std::chrono::steady_clock::now();
+                host_s = std::chrono::duration<double>(t1 - t0).count();
+                total_s = std::chrono::duration<double>(t2 - t0).count();
+            }
+            std::printf("{\"h2d\": \"%s\", \"bytes\": %zu, \"seconds\": %.4f, \"gb_per_s\": %.2f, "
+                        "\"seconds_in_calls\": %.4f}\n",
+                        k.name, 16 * blk, total_s, 16 * blk / total_s * 1e-9, host_s);
+            hipHostFree(h);
+        }
+        hipStreamDestroy(st);
+        hipFree(dev);
+        std::fflush(stdout);
+    }
     if (which == "chain" || which == "both") {
         // the reference's defaults: micro-batches of 4 over 8 streams per MODCOD
         rx_fec_pool pool(FECFRAME_NORMAL, ids, 25, 64, 8, 4);

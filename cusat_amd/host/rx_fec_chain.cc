@@ -136,6 +136,11 @@ rx_fec_chain::rx_fec_chain(fecframe_size_t framesize, code_rate_t rate, modulati
             f.valid = f.trials + mb;
             f.dfl = f.trials + 2 * mb;
             hip_check(hipHostMalloc((void**)&f.gse, mb * d_max_dfl, hipHostMallocDefault), "pinned slots");
+            // touch every page now: a first write into pinned memory costs a
+            // fault per page, which would land in submit_xfecframe
+            memset(f.syms, 0, mb * d_xfecframe_len * sizeof(complexf));
+            memset(f.trials, 0, 3 * mb * sizeof(int32_t));
+            memset(f.gse, 0, mb * d_max_dfl);
             hipEvent_t ev = nullptr;
             hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "events");
             f.event = ev;
@@ -235,7 +240,7 @@ bool rx_fec_chain::submit_xfecframe(const complexf* symbols, int n_symbols)
     memcpy(dst, symbols, (size_t)n_symbols * sizeof(complexf));
     if (++d_snr_skip_count >= 32) {
         d_snr_skip_count = 0;
-        update_dd_snr(dst);
+        update_dd_snr(symbols);  // the caller's copy: pinned memory is slow to read from the host
     }
     if (d_batch_idx == 0) d_batch_start_time = std::chrono::steady_clock::now();
     if (++d_batch_idx >= d_micro_batch) {

@@ -73,7 +73,6 @@ rx_fec_stream_pool::rx_fec_stream_pool(fecframe_size_t framesize, const std::vec
             for (size_t k = 0; k < pts.size() / 2; k++) s.points.emplace_back(pts[2 * k], pts[2 * k + 1]);
             d_max_xfec = std::max(d_max_xfec, s.xfec);
             d_max_dfl = std::max(d_max_dfl, s.max_dfl);
-            d_max_output_bytes = std::max(d_max_output_bytes, (s.max_dfl + 10) * N_FLIGHT * micro_batch);
             d_by_idx[idx] = (int)d_mc.size();
             d_mc.push_back(std::move(s));
         }
@@ -108,13 +107,23 @@ rx_fec_stream_pool::rx_fec_stream_pool(fecframe_size_t framesize, const std::vec
             hip_check(ei, "graph instantiate");
             r.graph = ge;
         }
-        d_flight.resize(N_FLIGHT);
+        // a ring of about N_FRAMES_BUFFERED frames (128 micro-batches of 4 in
+        // the reference's chain), at least 8 micro-batches
+        d_n_flight = std::max(8, (N_FRAMES_BUFFERED + micro_batch - 1) / micro_batch);
+        d_flight.resize(d_n_flight);
+        // an output block never exceeds the ring's frames of the largest code
+        d_max_output_bytes = (d_max_dfl + 10) * d_n_flight * micro_batch;
         for (auto& f : d_flight) {
             hip_check(hipHostMalloc((void**)&f.syms, mb * d_max_xfec * sizeof(complexf), hipHostMallocDefault),
                       "pinned slots");
             hip_check(hipHostMalloc((void**)&f.res, 3 * mb * sizeof(int32_t), hipHostMallocDefault), "pinned slots");
             hip_check(hipHostMalloc((void**)&f.out, std::max<size_t>(mb * d_max_dfl, 1), hipHostMallocDefault),
                       "pinned slots");
+            // touch every page now: a first write into pinned memory costs a
+            // fault per page, which would land in submit
+            memset(f.syms, 0, mb * d_max_xfec * sizeof(complexf));
+            memset(f.res, 0, 3 * mb * sizeof(int32_t));
+            memset(f.out, 0, std::max<size_t>(mb * d_max_dfl, 1));
             hipEvent_t ev = nullptr;
             hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "events");
             f.event = ev;
@@ -157,7 +166,7 @@ rx_fec_stream_pool::~rx_fec_stream_pool()
 {
     // frames still in flight hold budget that may be shared
     for (; d_drain_idx < d_launch_idx; d_drain_idx++) {
-        flight_slot& s = d_flight[d_drain_idx % N_FLIGHT];
+        flight_slot& s = d_flight[d_drain_idx % d_n_flight];
         if (s.in_flight) {
             hipEventSynchronize(s.event);
             release_budget(s.n_frames);
@@ -201,14 +210,14 @@ bool rx_fec_stream_pool::submit(const complexf* symbols, int n_symbols, const pl
     if (it == d_by_idx.end()) return false;
     modcod_state& m = d_mc[it->second];
     // drain at micro-batch boundaries and when the ring fills up
-    if (d_batch_idx == 0 || d_submit_idx - d_drain_idx >= N_FLIGHT / 2) drain(false);
-    if (d_submit_idx - d_drain_idx >= N_FLIGHT - 2 || n_symbols != m.xfec || !symbols) {
+    if (d_batch_idx == 0 || d_submit_idx - d_drain_idx >= d_n_flight / 2) drain(false);
+    if (d_submit_idx - d_drain_idx >= d_n_flight - 2 || n_symbols != m.xfec || !symbols) {
         d_gpu_drop_cnt++;
         m.frames++;
         m.errors++;
         return !d_ready.empty();
     }
-    flight_slot& s = d_flight[d_submit_idx % N_FLIGHT];
+    flight_slot& s = d_flight[d_submit_idx % d_n_flight];
     if (d_batch_idx == 0) {
         s.mc.clear();
         s.sym_off.clear();
@@ -220,7 +229,7 @@ bool rx_fec_stream_pool::submit(const complexf* symbols, int n_symbols, const pl
     memcpy(dst, symbols, (size_t)n_symbols * sizeof(complexf));
     if (++m.snr_skip >= 32) {
         m.snr_skip = 0;
-        update_dd_snr(m, dst);
+        update_dd_snr(m, symbols);  // the caller's copy: pinned memory is slow to read from the host
     }
     s.mc.push_back(it->second);
     s.sym_off.push_back(s.n_syms);
@@ -234,7 +243,7 @@ bool rx_fec_stream_pool::submit(const complexf* symbols, int n_symbols, const pl
 void rx_fec_stream_pool::close_batch()
 {
     if (d_batch_idx > 0) {
-        d_flight[d_submit_idx % N_FLIGHT].n_frames = d_batch_idx;
+        d_flight[d_submit_idx % d_n_flight].n_frames = d_batch_idx;
         d_submit_idx++;
         d_batch_idx = 0;
     }
@@ -244,7 +253,7 @@ void rx_fec_stream_pool::close_batch()
 void rx_fec_stream_pool::pump_launches(bool bypass_cap)
 {
     while (d_launch_idx < d_submit_idx) {
-        flight_slot& s = d_flight[d_launch_idx % N_FLIGHT];
+        flight_slot& s = d_flight[d_launch_idx % d_n_flight];
         if (!bypass_cap && d_budget && d_budget->cap > 0) {
             const int cur = d_budget->in_flight.load(std::memory_order_relaxed);
             // an idle GPU always admits, so an oversized micro-batch cannot stall
@@ -328,7 +337,7 @@ bool rx_fec_stream_pool::drain(bool wait_all)
         if (wait_all && d_drain_idx >= d_launch_idx && d_drain_idx < d_submit_idx)
             pump_launches(true);  // the oldest micro-batch may still wait for budget
         if (d_drain_idx >= d_launch_idx) break;
-        flight_slot& s = d_flight[d_drain_idx % N_FLIGHT];
+        flight_slot& s = d_flight[d_drain_idx % d_n_flight];
         if (s.in_flight) {
             if (wait_all) {
                 hip_check(hipEventSynchronize(s.event), "drain");  // not under the launch mutex
@@ -386,7 +395,7 @@ void rx_fec_stream_pool::reset_all()
         std::unique_lock<std::mutex> lk;
         if (d_mtx) lk = std::unique_lock<std::mutex>(*d_mtx);
         for (; d_drain_idx < d_launch_idx; d_drain_idx++) {
-            flight_slot& s = d_flight[d_drain_idx % N_FLIGHT];
+            flight_slot& s = d_flight[d_drain_idx % d_n_flight];
             if (s.in_flight) {
                 hip_check(hipEventSynchronize(s.event), "reset");
                 release_budget(s.n_frames);

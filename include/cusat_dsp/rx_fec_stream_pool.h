@@ -27,7 +27,8 @@
  *    streams (each with its own stream context and captured graph) overlap
  *    copies and launches; the decode budget caps frames in flight as
  *    gpu_decode_budget does; frames are dropped (and counted) only when the
- *    ring of 128 micro-batches is full;
+ *    ring is full: max(8, 512 / micro_batch) micro-batches, about the 512
+ *    frames of a chain's 128 micro-batches of 4;
  *  - HIP errors throw std::runtime_error.
  */
 #ifndef CUSAT_AMD_DSP_RX_FEC_STREAM_POOL_H
@@ -98,7 +99,7 @@ public:
     int pending_count() const { return d_batch_idx; }
 
 private:
-    static constexpr int N_FLIGHT = 128;
+    static constexpr int N_FRAMES_BUFFERED = 512;
 
     struct modcod_state {
         uint8_t idx = 0;
@@ -143,7 +144,7 @@ private:
     std::deque<block> d_ready;
     std::vector<int> d_last_sizes;
     int d_last_output = -1;                  // d_mc position of the latest output block
-    int d_micro_batch = 4, d_max_xfec = 0, d_max_dfl = 0, d_max_output_bytes = 0;
+    int d_micro_batch = 4, d_n_flight = 0, d_max_xfec = 0, d_max_dfl = 0, d_max_output_bytes = 0;
     int d_batch_idx = 0, d_submit_idx = 0, d_launch_idx = 0, d_drain_idx = 0, d_next_stream = 0;
     uint64_t d_gpu_drop_cnt = 0;
     bool d_diag[3] = {false, false, false};
