@@ -96,7 +96,8 @@ def lib():
         L.cusat_stream_destroy.argtypes = [vp]
         L.cusat_stream_plan.argtypes = [vp, vp, i, vp]
         L.cusat_stream_plan_async.argtypes = [vp, vp, i, vp]
-        L.cusat_stream_rx_plan_async.argtypes = [vp, vp, i, vp]
+        L.cusat_stream_rx_plan_async.argtypes = [vp, vp, i, vp, i, vp]
+        L.cusat_stream_rx_dd.argtypes = [vp, vp, vp, vp, vp]
         L.cusat_stream_rx.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.cusat_stream_get_tuning.argtypes = [vp, ctypes.POINTER(StreamTuning), ctypes.POINTER(i)]
         L.cusat_stream_decode.argtypes = [vp, vp, vp, vp, vp, vp, vp]
@@ -221,8 +222,12 @@ STREAM_ORDER_GIVEN, STREAM_ORDER_COST = 0, 1
 STREAM_FRAME = np.dtype([("code", np.int32), ("reserved", np.int32), ("llr_offset", np.int64),
                          ("hard_offset", np.int64)])
 # cusat_stream_rx_frame: a frame of a mixed receive batch
-STREAM_RX_FRAME = np.dtype([("code", np.int32), ("reserved", np.int32), ("sym_offset", np.int64),
+STREAM_RX_FRAME = np.dtype([("code", np.int32), ("flags", np.int32), ("sym_offset", np.int64),
                             ("out_offset", np.int64), ("noise_var", np.float32), ("llr_gain", np.float32)])
+RX_SCALE_STATE = 1
+# cusat_stream_rx_event: an SNR event of a receive batch
+STREAM_RX_EVENT = np.dtype([("code", np.int32), ("frame", np.int32), ("snr_lin", np.float32),
+                            ("snr_db", np.float32)])
 
 
 class StreamTuning(ctypes.Structure):
@@ -268,11 +273,18 @@ class StreamContext:
         _check(lib().cusat_stream_plan_async(self._h, _ptr(frames), len(frames), _stream(stream)),
                "cusat_stream_plan_async")
 
-    def rx_plan_async(self, frames, stream=None):
-        """Make a mixed receive batch (STREAM_RX_FRAME records) current."""
+    def rx_plan_async(self, frames, events=None, stream=None):
+        """Make a mixed receive batch (STREAM_RX_FRAME records, optional
+        STREAM_RX_EVENT SNR events) current."""
         frames = np.ascontiguousarray(frames, STREAM_RX_FRAME)
-        _check(lib().cusat_stream_rx_plan_async(self._h, _ptr(frames), len(frames), _stream(stream)),
-               "cusat_stream_rx_plan_async")
+        ev = np.ascontiguousarray(events if events is not None else np.zeros(0, STREAM_RX_EVENT), STREAM_RX_EVENT)
+        _check(lib().cusat_stream_rx_plan_async(self._h, _ptr(frames), len(frames), _ptr(ev), len(ev),
+                                                _stream(stream)), "cusat_stream_rx_plan_async")
+
+    def rx_dd(self, sym, state, state_out=None, stream=None):
+        """Apply the batch's SNR events to state (float32 [n_codes, 4], device)."""
+        _check(lib().cusat_stream_rx_dd(self._h, _ptr(sym), _ptr(state), _ptr(state_out), _stream(stream)),
+               "cusat_stream_rx_dd")
 
     def rx(self, sym, out, trials, valid, dfl, stream=None):
         """Demap, decode, BCH-check and BB-extract the current receive batch."""

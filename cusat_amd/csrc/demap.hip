@@ -154,7 +154,8 @@ __global__ void __launch_bounds__(256) demap_kernel(demap_args a)
 // serves any batch).  Every symbol gets exactly the arithmetic of demap_kernel.
 __global__ void __launch_bounds__(256)
     demap_mixed_kernel(const rx_code* __restrict__ codes, const rx_rec* __restrict__ recs,
-                       const int* __restrict__ hdr, const float* __restrict__ sym, int16_t* __restrict__ llr)
+                       const rx_scale* __restrict__ snap, const int* __restrict__ hdr,
+                       const float* __restrict__ sym, int16_t* __restrict__ llr)
 {
     __shared__ float pts[512];
     const int n = hdr[0], total = hdr[1];
@@ -177,9 +178,16 @@ __global__ void __launch_bounds__(256)
         a.n_points = c.n_points;
 #pragma unroll
         for (int b = 0; b < 8; b++) a.col[b] = c.col[b];
-        a.qpsk_scale = r.qpsk_scale;
-        a.psk8_scale = r.psk8_scale;
-        a.gain = r.gain;
+        if (r.flags & RX_SCALE_STATE) {
+            const rx_scale& sc = snap[r.code];
+            a.qpsk_scale = sc.qpsk_scale;
+            a.psk8_scale = sc.psk8_scale;
+            a.gain = sc.gain;
+        } else {
+            a.qpsk_scale = r.qpsk_scale;
+            a.psk8_scale = r.psk8_scale;
+            a.gain = r.gain;
+        }
         if (c.pts && r.code != cur) {
             __syncthreads();  // every lane is done with the previous table
             for (int i = threadIdx.x; i < 2 * c.n_points; i += blockDim.x) pts[i] = c.pts[i];
@@ -192,6 +200,165 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// Decision-directed SNR of one frame, the host estimate's arithmetic
+// exactly (rx_fec_chain.cc qpsk_snr_dd / constellation_snr_dd, after
+// modulation/dd_snr.cc:23-51): QPSK over every symbol with sign decisions,
+// other constellations over every 8th symbol with the nearest point; the
+// sums run in symbol order in float.  The 256 lanes compute the per-symbol
+// terms of a 256-symbol chunk into the LDS; lanes 0 and 1 then add the
+// chunk to the signal and error sums, one term after the other, as the
+// host loop does (lane 0 adds +0.0f after each signal term, an identity
+// for sums that are never -0, so both lanes run one instruction stream).
+__device__ __forceinline__ float dd_ratio(float signal, float error)
+{
+    if (!isfinite(signal) || !isfinite(error)) return 0.0f;
+    const float cap = 1e6f;  // error-free input reads as 60 dB
+    return error * cap <= signal ? cap : signal / error;
+}
+
+__global__ void __launch_bounds__(256)
+    dd_estimate_kernel(const rx_code* __restrict__ codes, const rx_rec* __restrict__ recs,
+                       const rx_event* __restrict__ ev, const int* __restrict__ hdr,
+                       const float* __restrict__ sym, float* __restrict__ est)
+{
+    __shared__ float pts[512];
+    __shared__ __align__(16) float term_a[2][256];  // [0]: signal, [1]: error
+    __shared__ __align__(16) float term_b[2][256];
+    const int n_ev = hdr[2];
+    const int t = threadIdx.x;
+    for (int e = blockIdx.x; e < n_ev; e += gridDim.x) {
+        const int f = ev[e].frame;
+        if (f < 0) continue;
+        const rx_rec& r = recs[f];
+        const rx_code& c = codes[r.code];
+        const bool qpsk = c.n_mod == 2;
+        const int np = c.n_points;
+        __syncthreads();  // the previous frame's LDS reads are done
+        if (!qpsk)
+            for (int i = t; i < 2 * np; i += 256) pts[i] = c.dd_pts[i];
+        __syncthreads();
+        const float2* y = reinterpret_cast<const float2*>(sym) + r.sym_off;
+        const int stride = qpsk ? 1 : 8;
+        const int m = (c.xfec + stride - 1) / stride;
+        float acc = 0.0f;  // lane 0: signal, lane 1: error
+        for (int c0 = 0; c0 < m; c0 += 256) {
+            const int j = c0 + t;
+            if (j < m) {
+                const float2 v = y[(size_t)j * stride];
+                float sig, e1, e2;
+                if (qpsk) {
+                    const float a = 0.70710678118654752f;
+                    const float dr = v.x < 0 ? -a : a, di = v.y < 0 ? -a : a;
+                    sig = dr * dr + di * di;
+                    e1 = (v.x - dr) * (v.x - dr);
+                    e2 = (v.y - di) * (v.y - di);
+                } else {
+                    int best = 0;
+                    float xr = v.x - pts[0], xi = v.y - pts[1];
+                    float bd = xr * xr + xi * xi;
+                    for (int k = 1; k < np; k++) {
+                        xr = v.x - pts[2 * k];
+                        xi = v.y - pts[2 * k + 1];
+                        const float d = xr * xr + xi * xi;
+                        if (d < bd) {
+                            bd = d;
+                            best = k;
+                        }
+                    }
+                    const float pr = pts[2 * best], pi = pts[2 * best + 1];
+                    sig = pr * pr + pi * pi;
+                    e1 = bd;
+                    e2 = 0.0f;
+                }
+                term_a[0][t] = sig;
+                term_b[0][t] = 0.0f;
+                term_a[1][t] = e1;
+                term_b[1][t] = e2;
+            }
+            __syncthreads();
+            if (t < 2) {
+                const int n = min(256, m - c0);
+                const float* A = term_a[t];
+                const float* B = term_b[t];
+                int k = 0;
+                // eight terms per 16-byte LDS reads, the next eight loaded
+                // before these are added: the adds stay in symbol order
+                if (n >= 8) {
+                    float4 a0 = *reinterpret_cast<const float4*>(A), a1 = *reinterpret_cast<const float4*>(A + 4);
+                    float4 b0 = *reinterpret_cast<const float4*>(B), b1 = *reinterpret_cast<const float4*>(B + 4);
+                    for (; k + 8 <= n; k += 8) {
+                        float4 na0 = a0, na1 = a1, nb0 = b0, nb1 = b1;
+                        if (k + 16 <= n) {
+                            na0 = *reinterpret_cast<const float4*>(A + k + 8);
+                            na1 = *reinterpret_cast<const float4*>(A + k + 12);
+                            nb0 = *reinterpret_cast<const float4*>(B + k + 8);
+                            nb1 = *reinterpret_cast<const float4*>(B + k + 12);
+                        }
+                        acc = (acc + a0.x) + b0.x;
+                        acc = (acc + a0.y) + b0.y;
+                        acc = (acc + a0.z) + b0.z;
+                        acc = (acc + a0.w) + b0.w;
+                        acc = (acc + a1.x) + b1.x;
+                        acc = (acc + a1.y) + b1.y;
+                        acc = (acc + a1.z) + b1.z;
+                        acc = (acc + a1.w) + b1.w;
+                        a0 = na0;
+                        a1 = na1;
+                        b0 = nb0;
+                        b1 = nb1;
+                    }
+                }
+                for (; k < n; k++) acc = (acc + A[k]) + B[k];
+            }
+            __syncthreads();
+        }
+        const float error = __shfl(acc, 1);
+        if (t == 0) est[e] = dd_ratio(acc, error);
+    }
+}
+
+// One thread applies the batch's events in order (a host value, or an
+// estimate above 1: noise_var = 1 / snr, llr_gain = min(4 / noise_var,
+// 1000), as rx_fec_chain::update_snr / update_dd_snr), then every code's
+// demapper scales are computed from its state as cusat_fec_demap does.
+__global__ void __launch_bounds__(64)
+    dd_resolve_kernel(const rx_code* __restrict__ codes, const rx_event* __restrict__ ev,
+                      const float* __restrict__ est, const int* __restrict__ hdr, int n_codes, float4* state,
+                      rx_scale* __restrict__ snap, float4* __restrict__ state_out)
+{
+    if (threadIdx.x == 0) {
+        const int n_ev = hdr[2];
+        for (int e = 0; e < n_ev; e++) {
+            const rx_event x = ev[e];
+            float s, db;
+            if (x.frame < 0) {
+                s = x.snr_lin;
+                db = x.snr_db;
+            } else {
+                s = est[e];
+                if (!(s > 1.0f)) continue;
+                db = __builtin_nanf("");  // the host takes 10 log10(snr_lin)
+            }
+            const float nv = 1.0f / s;
+            state[x.code] = make_float4(nv, fminf(4.0f / nv, 1000.0f), s, db);
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < n_codes; k += blockDim.x) {
+        const float4 st = state[k];
+        const rx_code& c = codes[k];
+        rx_scale o;
+        o.qpsk_scale = 2.0f * 1.4142135623730951f / st.x;
+        float gain = st.y;
+        if (c.has_cap) gain = st.y < c.gain_cap ? st.y : c.gain_cap;
+        o.gain = gain;
+        o.psk8_scale = 2.0f * 0.38268343236508977f * gain;
+        o.snr_lin = st.z;
+        snap[k] = o;
+        if (state_out) state_out[k] = st;
+    }
+}
+
 }  // namespace
 
 hipError_t launch_demap(const demap_args& a, int batch, hipStream_t stream)
@@ -201,10 +368,25 @@ hipError_t launch_demap(const demap_args& a, int batch, hipStream_t stream)
     return hipGetLastError();
 }
 
-hipError_t launch_demap_mixed(const rx_code* codes, const rx_rec* recs, const int* hdr, const float* sym,
-                              int16_t* llr, int grid, hipStream_t stream)
+hipError_t launch_demap_mixed(const rx_code* codes, const rx_rec* recs, const rx_scale* snap, const int* hdr,
+                              const float* sym, int16_t* llr, int grid, hipStream_t stream)
 {
-    hipLaunchKernelGGL(demap_mixed_kernel, dim3(grid), dim3(256), 0, stream, codes, recs, hdr, sym, llr);
+    hipLaunchKernelGGL(demap_mixed_kernel, dim3(grid), dim3(256), 0, stream, codes, recs, snap, hdr, sym, llr);
+    return hipGetLastError();
+}
+
+hipError_t launch_dd_estimate(const rx_code* codes, const rx_rec* recs, const rx_event* ev, const int* hdr,
+                              const float* sym, float* est, int grid, hipStream_t stream)
+{
+    hipLaunchKernelGGL(dd_estimate_kernel, dim3(grid), dim3(256), 0, stream, codes, recs, ev, hdr, sym, est);
+    return hipGetLastError();
+}
+
+hipError_t launch_dd_resolve(const rx_code* codes, const rx_event* ev, const float* est, const int* hdr,
+                             int n_codes, float4* state, rx_scale* snap, float4* state_out, hipStream_t stream)
+{
+    hipLaunchKernelGGL(dd_resolve_kernel, dim3(1), dim3(64), 0, stream, codes, ev, est, hdr, n_codes, state, snap,
+                       state_out);
     return hipGetLastError();
 }
 

@@ -66,8 +66,11 @@ hipError_t launch_demap(const demap_args& a, int batch, hipStream_t stream);
 struct rx_code {
     const float* pts;          // constellation (APSK), null for the QPSK / 8PSK formulas
     const uint8_t* prbs;       // BB scrambler sequence, kbch_bytes
+    const float* dd_pts;       // constellation of every code (decision-directed SNR)
     int xfec, dlen, N, n_mod, n_points;
     int kbch_bytes, hard_bytes;  // k_bch / 8, k_ldpc / 8
+    int has_cap;               // the demapper gain is capped at gain_cap
+    float gain_cap;
     int pad;
     uint32_t col[8];
 };
@@ -77,11 +80,33 @@ struct rx_rec {
     int64_t llr_off;           // LLRs
     int64_t hard_off;          // bytes
     int64_t out_off;           // data-field bytes
-    float qpsk_scale, psk8_scale, gain, pad2;
+    float qpsk_scale, psk8_scale, gain;
+    int flags;                 // RX_SCALE_STATE: scales from the batch's state snapshot
 };
-// hdr[0] = frames, hdr[1] = demap blocks of the batch (device memory)
-hipError_t launch_demap_mixed(const rx_code* codes, const rx_rec* recs, const int* hdr, const float* sym,
-                              int16_t* llr, int grid, hipStream_t stream);
+constexpr int RX_SCALE_STATE = 1;
+// SNR events of a receive batch, applied in order to the carried per-code
+// state: frame >= 0 = the decision-directed estimate from that frame's
+// symbols (applied when above 1), frame < 0 = the value snr_lin (snr_db
+// kept for reporting)
+struct rx_event {
+    int code, frame;
+    float snr_lin, snr_db;
+};
+// per-code demapper scales of a batch (state after its events)
+struct rx_scale {
+    float qpsk_scale, psk8_scale, gain, snr_lin;
+};
+// hdr[0] = frames, hdr[1] = demap blocks, hdr[2] = SNR events of the batch (device memory)
+hipError_t launch_demap_mixed(const rx_code* codes, const rx_rec* recs, const rx_scale* snap, const int* hdr,
+                              const float* sym, int16_t* llr, int grid, hipStream_t stream);
+// est[e] = decision-directed SNR of the frame of every estimate event
+hipError_t launch_dd_estimate(const rx_code* codes, const rx_rec* recs, const rx_event* ev, const int* hdr,
+                              const float* sym, float* est, int grid, hipStream_t stream);
+// apply the events in order to state[n_codes] (noise_var, llr_gain, snr_lin,
+// snr_db); write the batch's snapshot and, when state_out is not null, a copy
+// of the new state
+hipError_t launch_dd_resolve(const rx_code* codes, const rx_event* ev, const float* est, const int* hdr,
+                             int n_codes, float4* state, rx_scale* snap, float4* state_out, hipStream_t stream);
 hipError_t launch_bb_extract_mixed(const rx_code* codes, const rx_rec* recs, const int* hdr, const uint8_t* hard,
                                    const int32_t* trials, const int32_t* valid, uint8_t* out, int32_t* dfl_out,
                                    int grid, hipStream_t stream);

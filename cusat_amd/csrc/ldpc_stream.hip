@@ -295,7 +295,10 @@ struct cusat_stream_ctx {
     // plan region, LLR and hard-decision work buffers (allocated by the first
     // rx plan)
     rx_code* d_rx_codes = nullptr;
-    size_t off_rx = 0;
+    size_t off_rx = 0, off_ev = 0;  // rx header + records; SNR events
+    int max_events = 0;
+    rx_scale* d_snap = nullptr;     // per-code demapper scales of the current batch
+    float* d_est = nullptr;         // per-event decision-directed estimates
     int64_t llr_max = 0, hard_max = 0;  // per frame
     int16_t* d_llr_work = nullptr;
     uint8_t* d_hard_work = nullptr;
@@ -414,6 +417,9 @@ int upload_codes(cusat_stream_ctx* sc)
         const bool formula = c->modulation == CUSAT_MOD_QPSK || c->modulation == CUSAT_MOD_8PSK;
         o.pts = formula ? nullptr : c->d_pts;
         o.prbs = c->d_prbs;
+        o.dd_pts = c->d_pts;
+        o.has_cap = c->llr_target > 0.0f && c->dmax > 0.0f;
+        o.gain_cap = o.has_cap ? 2.0f * c->llr_target / c->dmax : 0.0f;  // as cusat_fec_demap
         o.xfec = c->xfec;
         o.dlen = c->dlen;
         o.N = c->geo.N;
@@ -525,7 +531,9 @@ extern "C" int cusat_stream_create(cusat_stream_ctx** out, cusat_fec_ctx* const*
     sc->off_jobs = 256 + al(sizeof(cusat_stream_frame) * max_frames);
     sc->off_order = sc->off_jobs + al(sizeof(int2) * max_frames);
     sc->off_rx = sc->off_order + al(sizeof(int) * max_frames);
-    sc->plan_bytes = sc->off_rx + 256 + al(sizeof(rx_rec) * max_frames);
+    sc->max_events = 2 * max_frames + n_codes;
+    sc->off_ev = sc->off_rx + 256 + al(sizeof(rx_rec) * max_frames);
+    sc->plan_bytes = sc->off_ev + al(sizeof(rx_event) * sc->max_events);
     const size_t bytes = sc->plan_bytes + al(sizeof(code_desc) * n_codes) + al(sizeof(rx_code) * n_codes);
     char* base = nullptr;
     e = hipMalloc(&base, bytes);
@@ -575,6 +583,8 @@ extern "C" int cusat_stream_destroy(cusat_stream_ctx* sc)
     if (sc->d_counters) hipFree(sc->d_counters);
     if (sc->d_ws) hipFree(sc->d_ws);
     if (sc->d_llr_work) hipFree(sc->d_llr_work);
+    if (sc->d_snap) hipFree(sc->d_snap);
+    if (sc->d_est) hipFree(sc->d_est);
     if (sc->d_hard_work) hipFree(sc->d_hard_work);
     delete sc;
     return CUSAT_OK;
@@ -689,18 +699,31 @@ extern "C" int cusat_stream_decode(cusat_stream_ctx* sc, const int16_t* llr, uin
 }
 
 extern "C" int cusat_stream_rx_plan_async(cusat_stream_ctx* sc, const cusat_stream_rx_frame* frames, int n_frames,
-                                          void* stream)
+                                          const cusat_stream_rx_event* events, int n_events, void* stream)
 {
-    if (!sc || (n_frames > 0 && !frames) || n_frames < 0 || n_frames > sc->max_frames) {
-        set_last_error("bad argument (n_frames %d, max %d)", n_frames, sc ? sc->max_frames : 0);
+    if (!sc || (n_frames > 0 && !frames) || n_frames < 0 || n_frames > sc->max_frames || (n_events > 0 && !events) ||
+        n_events < 0 || n_events > sc->max_events) {
+        set_last_error("bad argument (n_frames %d, max %d; n_events %d, max %d)", n_frames, sc ? sc->max_frames : 0,
+                       n_events, sc ? sc->max_events : 0);
         return CUSAT_E_INVALID;
     }
-    for (int k = 0; k < n_frames; k++)
+    for (int k = 0; k < n_frames; k++) {
+        const bool state = frames[k].flags & CUSAT_RX_SCALE_STATE;
         if (frames[k].code < 0 || frames[k].code >= (int)sc->codes.size() || frames[k].sym_offset < 0 ||
-            frames[k].out_offset < 0 || !(frames[k].noise_var > 0.0f)) {
-            set_last_error("rx frame %d: code %d / offsets / noise_var out of range", k, frames[k].code);
+            frames[k].out_offset < 0 || (frames[k].flags & ~CUSAT_RX_SCALE_STATE) ||
+            (!state && !(frames[k].noise_var > 0.0f))) {
+            set_last_error("rx frame %d: code %d / offsets / flags / noise_var out of range", k, frames[k].code);
             return CUSAT_E_INVALID;
         }
+    }
+    for (int k = 0; k < n_events; k++) {
+        const cusat_stream_rx_event& e = events[k];
+        if (e.code < 0 || e.code >= (int)sc->codes.size() || e.frame < -1 || e.frame >= n_frames ||
+            (e.frame < 0 && !(e.snr_lin > 0.0f && e.snr_lin < 1e30f))) {
+            set_last_error("rx event %d: code %d / frame %d / snr out of range", k, e.code, e.frame);
+            return CUSAT_E_INVALID;
+        }
+    }
     if (!sc->codes_uploaded) {
         const int rc = upload_codes(sc);
         if (rc != CUSAT_OK) return rc;
@@ -709,6 +732,9 @@ extern "C" int cusat_stream_rx_plan_async(cusat_stream_ctx* sc, const cusat_stre
     if (!sc->d_llr_work) {
         HIP_TRY(hipMalloc(&sc->d_llr_work, (size_t)sc->max_frames * sc->llr_max * sizeof(int16_t)));
         HIP_TRY(hipMalloc(&sc->d_hard_work, (size_t)sc->max_frames * sc->hard_max));
+        HIP_TRY(hipMalloc(&sc->d_snap, sc->codes.size() * sizeof(rx_scale)));
+        HIP_TRY(hipMalloc(&sc->d_est, sc->max_events * sizeof(float)));
+        HIP_TRY(hipMemset(sc->d_snap, 0, sc->codes.size() * sizeof(rx_scale)));
     }
     const int k = sc->next_stage;
     if (!sc->stage[k]) {
@@ -734,7 +760,11 @@ extern "C" int cusat_stream_rx_plan_async(cusat_stream_ctx* sc, const cusat_stre
         r.llr_off = lo;
         r.hard_off = ho;
         r.out_off = frames[f].out_offset;
-        cusat_fec_demap_scales(c, frames[f].noise_var, frames[f].llr_gain, &r.qpsk_scale, &r.psk8_scale, &r.gain);
+        if (frames[f].flags & CUSAT_RX_SCALE_STATE)
+            r.flags = RX_SCALE_STATE;
+        else
+            cusat_fec_demap_scales(c, frames[f].noise_var, frames[f].llr_gain, &r.qpsk_scale, &r.psk8_scale,
+                                   &r.gain);
         lo += c->geo.N;
         ho += c->geo.K / 8;
         blk += (c->dlen + 255) / 256;
@@ -742,12 +772,19 @@ extern "C" int cusat_stream_rx_plan_async(cusat_stream_ctx* sc, const cusat_stre
     memset(hdr, 0, 256);
     hdr[0] = n_frames;
     hdr[1] = blk;
+    hdr[2] = n_events;
+    rx_event* ev = reinterpret_cast<rx_event*>(img + sc->off_ev);
+    for (int e = 0; e < n_events; e++)
+        ev[e] = rx_event{events[e].code, events[e].frame, events[e].snr_lin, events[e].snr_db};
     sc->n_frames = n_frames;
     const int nj = stage_plan(sc, sc->tuning, img);
     hipStream_t st = (hipStream_t)stream;
     HIP_TRY(copy_plan(sc, img, n_frames, nj, st, true));
     HIP_TRY(hipMemcpyAsync(reinterpret_cast<unsigned char*>(sc->d_counters) + sc->off_rx, img + sc->off_rx,
                            256 + n_frames * sizeof(rx_rec), hipMemcpyHostToDevice, st));
+    if (n_events)
+        HIP_TRY(hipMemcpyAsync(reinterpret_cast<unsigned char*>(sc->d_counters) + sc->off_ev, img + sc->off_ev,
+                               n_events * sizeof(rx_event), hipMemcpyHostToDevice, st));
     HIP_TRY(hipEventRecord(sc->stage_done[k], st));
     sc->stage_used[k] = true;
     sc->next_stage = (k + 1) % cusat_stream_ctx::N_STAGE;
@@ -767,10 +804,29 @@ extern "C" int cusat_stream_rx(cusat_stream_ctx* sc, const float* sym, uint8_t* 
     hipStream_t st = (hipStream_t)stream;
     const int* hdr = reinterpret_cast<const int*>(reinterpret_cast<unsigned char*>(sc->d_counters) + sc->off_rx);
     const rx_rec* rec = reinterpret_cast<const rx_rec*>(hdr + 64);
-    HIP_TRY(launch_demap_mixed(sc->d_rx_codes, rec, hdr, sym, sc->d_llr_work, sc->n_cu * 4, st));
+    HIP_TRY(launch_demap_mixed(sc->d_rx_codes, rec, sc->d_snap, hdr, sym, sc->d_llr_work, sc->n_cu * 4, st));
     HIP_TRY(launch_stream(sc, sc->d_llr_work, sc->d_hard_work, trials, valid, nullptr, stream));
     HIP_TRY(launch_bb_extract_mixed(sc->d_rx_codes, rec, hdr, sc->d_hard_work, trials, valid, out, dfl, sc->n_cu * 2,
                                     st));
+    return CUSAT_OK;
+}
+
+extern "C" int cusat_stream_rx_dd(cusat_stream_ctx* sc, const float* sym, float* state, float* state_out,
+                                  void* stream)
+{
+    if (!sc || !sym || !state) { set_last_error("bad argument"); return CUSAT_E_INVALID; }
+    if (!sc->rx_planned) {
+        set_last_error("cusat_stream_rx_dd needs an rx plan (cusat_stream_rx_plan_async)");
+        return CUSAT_E_INVALID;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    const unsigned char* base = reinterpret_cast<const unsigned char*>(sc->d_counters);
+    const int* hdr = reinterpret_cast<const int*>(base + sc->off_rx);
+    const rx_rec* rec = reinterpret_cast<const rx_rec*>(hdr + 64);
+    const rx_event* ev = reinterpret_cast<const rx_event*>(base + sc->off_ev);
+    HIP_TRY(launch_dd_estimate(sc->d_rx_codes, rec, ev, hdr, sym, sc->d_est, 64, st));
+    HIP_TRY(launch_dd_resolve(sc->d_rx_codes, ev, sc->d_est, hdr, (int)sc->codes.size(),
+                              reinterpret_cast<float4*>(state), sc->d_snap, reinterpret_cast<float4*>(state_out), st));
     return CUSAT_OK;
 }
 

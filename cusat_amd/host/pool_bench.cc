@@ -10,7 +10,10 @@
 // output collection and finish_all_pending, from host symbols to GSE bytes
 // (host copy into pinned slots, H2D, demap, LDPC, BCH, BB, D2H included).
 //
-// usage: pool_bench [frames] [run] [pool: chain|stream|both] [micro_batch] [streams] [modcods,comma-separated]
+// usage: pool_bench [frames] [run] [pool: chain|stream|stream_hostdd|both] [micro_batch] [streams]
+//                   [modcods,comma-separated]
+// The stream pool waits when its ring is full (set_block_when_full), so
+// its rate is what it sustains; stream_hostdd estimates the SNR on the host.
 // prints one JSON line per pool.
 #include <chrono>
 #include <cstdio>
@@ -233,9 +236,13 @@ int main(int argc, char** argv)
         rx_fec_pool pool(FECFRAME_NORMAL, ids, 25, 64, 8, 4);
         run(pool, "rx_fec_pool", src, arr, run_len, 4, 8);
     }
-    if (which == "stream" || which == "both") {
+    if (which == "stream" || which == "stream_hostdd" || which == "both") {
         rx_fec_stream_pool pool(FECFRAME_NORMAL, ids, 25, 64, streams, micro_batch);
-        run(pool, "rx_fec_stream_pool", src, arr, run_len, micro_batch, streams);
+        pool.set_block_when_full(true);
+        const bool host_dd = which == "stream_hostdd";
+        pool.set_gpu_dd_snr(!host_dd);
+        run(pool, host_dd ? "rx_fec_stream_pool (host DD-SNR)" : "rx_fec_stream_pool", src, arr, run_len,
+            micro_batch, streams);
     }
     return 0;
 }

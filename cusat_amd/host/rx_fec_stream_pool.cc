@@ -77,6 +77,14 @@ rx_fec_stream_pool::rx_fec_stream_pool(fecframe_size_t framesize, const std::vec
             d_mc.push_back(std::move(s));
         }
         const size_t mb = (size_t)micro_batch;
+        const size_t state_bytes = std::max<size_t>(d_mc.size(), 1) * 4 * sizeof(float);
+        hip_check(hipMalloc(&d_state, state_bytes), "state");
+        {
+            hipEvent_t ev = nullptr;
+            hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "events");
+            d_dd_done = ev;
+        }
+        reset_state();
         d_streams.resize(n_gpu_streams);
         for (auto& r : d_streams) {
             hipStream_t st = nullptr;
@@ -87,25 +95,30 @@ rx_fec_stream_pool::rx_fec_stream_pool(fecframe_size_t framesize, const std::vec
             hip_check(hipMalloc(&r.sym_dev, mb * d_max_xfec * sizeof(complexf)), "device buffers");
             hip_check(hipMalloc(&r.out_dev, mb * d_max_dfl), "device buffers");
             hip_check(hipMalloc(&r.res_dev, 3 * mb * sizeof(int32_t)), "device buffers");
-            // capture the three launches once; every micro-batch replays them
-            abi_check(cusat_stream_rx_plan_async(r.sc, nullptr, 0, st), "plan");
+            hip_check(hipMalloc(&r.state_dev, state_bytes), "device buffers");
+            // capture the SNR launch and the three receive launches once;
+            // every micro-batch replays them
+            abi_check(cusat_stream_rx_plan_async(r.sc, nullptr, 0, nullptr, 0, st), "plan");
             int32_t* res = r.res_dev;
-            abi_check(cusat_stream_rx(r.sc, reinterpret_cast<const float*>(r.sym_dev), r.out_dev, res, res + mb,
-                                      res + 2 * mb, st),
-                      "rx");
+            const float* sym = reinterpret_cast<const float*>(r.sym_dev);
+            auto rx = [&] { return cusat_stream_rx(r.sc, sym, r.out_dev, res, res + mb, res + 2 * mb, st); };
+            auto dd = [&] { return cusat_stream_rx_dd(r.sc, sym, d_state, r.state_dev, st); };
+            abi_check(dd(), "snr");
+            abi_check(rx(), "rx");
             hip_check(hipStreamSynchronize(st), "warm launch");
-            hipGraph_t g = nullptr;
-            hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "capture");
-            const int rc = cusat_stream_rx(r.sc, reinterpret_cast<const float*>(r.sym_dev), r.out_dev, res, res + mb,
-                                           res + 2 * mb, st);
-            const hipError_t ec = hipStreamEndCapture(st, &g);
-            abi_check(rc, "rx capture");
-            hip_check(ec, "capture");
-            hipGraphExec_t ge = nullptr;
-            const hipError_t ei = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-            hipGraphDestroy(g);
-            hip_check(ei, "graph instantiate");
-            r.graph = ge;
+            for (int k = 0; k < 2; k++) {
+                hipGraph_t g = nullptr;
+                hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "capture");
+                const int rc = k ? rx() : dd();
+                const hipError_t ec = hipStreamEndCapture(st, &g);
+                abi_check(rc, "capture");
+                hip_check(ec, "capture");
+                hipGraphExec_t ge = nullptr;
+                const hipError_t ei = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+                hipGraphDestroy(g);
+                hip_check(ei, "graph instantiate");
+                (k ? r.graph : r.graph_dd) = ge;
+            }
         }
         // a ring of about N_FRAMES_BUFFERED frames (128 micro-batches of 4 in
         // the reference's chain), at least 8 micro-batches
@@ -124,6 +137,9 @@ rx_fec_stream_pool::rx_fec_stream_pool(fecframe_size_t framesize, const std::vec
             memset(f.syms, 0, mb * d_max_xfec * sizeof(complexf));
             memset(f.res, 0, 3 * mb * sizeof(int32_t));
             memset(f.out, 0, std::max<size_t>(mb * d_max_dfl, 1));
+            hip_check(hipHostMalloc((void**)&f.state, state_bytes, hipHostMallocDefault), "pinned slots");
+            memset(f.state, 0, state_bytes);
+            f.events.reserve(2 * mb + d_mc.size());
             hipEvent_t ev = nullptr;
             hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "events");
             f.event = ev;
@@ -146,11 +162,14 @@ void rx_fec_stream_pool::free_all()
         if (f.syms) hipHostFree(f.syms);
         if (f.res) hipHostFree(f.res);
         if (f.out) hipHostFree(f.out);
+        if (f.state) hipHostFree(f.state);
         if (f.event) hipEventDestroy(f.event);
         f = flight_slot{};
     }
     for (auto& r : d_streams) {
         if (r.graph) hipGraphExecDestroy(r.graph);
+        if (r.graph_dd) hipGraphExecDestroy(r.graph_dd);
+        if (r.state_dev) hipFree(r.state_dev);
         if (r.sc) cusat_stream_destroy(r.sc);
         if (r.sym_dev) hipFree(r.sym_dev);
         if (r.out_dev) hipFree(r.out_dev);
@@ -160,6 +179,10 @@ void rx_fec_stream_pool::free_all()
     }
     for (cusat_fec_ctx* c : d_ctx) cusat_fec_ctx_destroy(c);
     d_ctx.clear();
+    if (d_state) hipFree(d_state);
+    d_state = nullptr;
+    if (d_dd_done) hipEventDestroy(d_dd_done);
+    d_dd_done = nullptr;
 }
 
 rx_fec_stream_pool::~rx_fec_stream_pool()
@@ -202,6 +225,37 @@ void rx_fec_stream_pool::update_snr(uint8_t modcod, float snr_db)
     m.snr_db = snr_db;
     m.noise_var = 1.0f / snr_lin;
     m.llr_gain = std::fmin(4.0f / m.noise_var, 1000.0f);
+    if (d_gpu_dd) add_host_event(m, snr_lin, snr_db);
+}
+
+void rx_fec_stream_pool::add_host_event(const modcod_state& m, float snr_lin, float snr_db)
+{
+    d_mc[m.code].host_event_batch = d_submit_idx;
+    // a host value that no estimate of the MODCOD follows yet is replaced,
+    // which keeps a micro-batch's events within 2 x frames + MODCODs
+    for (auto e = d_events.rbegin(); e != d_events.rend(); ++e) {
+        if (e->code != m.code) continue;
+        if (e->frame < 0) {
+            e->snr_lin = snr_lin;
+            e->snr_db = snr_db;
+            return;
+        }
+        break;
+    }
+    d_events.push_back(cusat_stream_rx_event{m.code, -1, snr_lin, snr_db});
+}
+
+void rx_fec_stream_pool::reset_state()
+{
+    std::vector<float> init(std::max<size_t>(d_mc.size(), 1) * 4, 0.0f);
+    for (size_t k = 0; k < d_mc.size(); k++) {
+        init[4 * k] = 1.0f;  // noise_var, llr_gain of a fresh chain
+        init[4 * k + 1] = 4.0f;
+    }
+    hip_check(hipMemcpy(d_state, init.data(), init.size() * sizeof(float), hipMemcpyHostToDevice), "state");
+    d_events.clear();
+    d_dd_recorded = false;
+    for (auto& m : d_mc) m.host_event_batch = -1;
 }
 
 bool rx_fec_stream_pool::submit(const complexf* symbols, int n_symbols, const pl_layout& pls)
@@ -211,6 +265,7 @@ bool rx_fec_stream_pool::submit(const complexf* symbols, int n_symbols, const pl
     modcod_state& m = d_mc[it->second];
     // drain at micro-batch boundaries and when the ring fills up
     if (d_batch_idx == 0 || d_submit_idx - d_drain_idx >= d_n_flight / 2) drain(false);
+    while (d_block_when_full && d_submit_idx - d_drain_idx >= d_n_flight - 2) drain(true, true);
     if (d_submit_idx - d_drain_idx >= d_n_flight - 2 || n_symbols != m.xfec || !symbols) {
         d_gpu_drop_cnt++;
         m.frames++;
@@ -229,7 +284,14 @@ bool rx_fec_stream_pool::submit(const complexf* symbols, int n_symbols, const pl
     memcpy(dst, symbols, (size_t)n_symbols * sizeof(complexf));
     if (++m.snr_skip >= 32) {
         m.snr_skip = 0;
-        update_dd_snr(m, symbols);  // the caller's copy: pinned memory is slow to read from the host
+        if (!d_gpu_dd) {
+            update_dd_snr(m, symbols);  // the caller's copy: pinned memory is slow to read from the host
+        } else if (m.n_mod >= 7 && m.pilot_snr_db > 5.0f) {  // dense APSK: the pilot estimate, on the host
+            const float snr_lin = std::pow(10.0f, m.pilot_snr_db / 10.0f);
+            add_host_event(m, snr_lin, 10.0f * std::log10(snr_lin));
+        } else {
+            d_events.push_back(cusat_stream_rx_event{m.code, d_batch_idx, 0.0f, 0.0f});
+        }
     }
     s.mc.push_back(it->second);
     s.sym_off.push_back(s.n_syms);
@@ -243,7 +305,11 @@ bool rx_fec_stream_pool::submit(const complexf* symbols, int n_symbols, const pl
 void rx_fec_stream_pool::close_batch()
 {
     if (d_batch_idx > 0) {
-        d_flight[d_submit_idx % d_n_flight].n_frames = d_batch_idx;
+        flight_slot& s = d_flight[d_submit_idx % d_n_flight];
+        s.events.swap(d_events);
+        d_events.clear();
+        s.n_frames = d_batch_idx;
+        s.batch = d_submit_idx;
         d_submit_idx++;
         d_batch_idx = 0;
     }
@@ -279,10 +345,10 @@ void rx_fec_stream_pool::launch_slot(flight_slot& s)
     for (int i = 0; i < n; i++) {
         const modcod_state& m = d_mc[s.mc[i]];
         rec[i].code = m.code;
-        rec[i].reserved = 0;
+        rec[i].flags = d_gpu_dd ? CUSAT_RX_SCALE_STATE : 0;  // the state after the batch's events
         rec[i].sym_offset = s.sym_off[i];
         rec[i].out_offset = s.out_off[i];
-        rec[i].noise_var = m.noise_var;  // the MODCOD's scaling at launch, as a chain's
+        rec[i].noise_var = m.noise_var;  // host mode: the MODCOD's scaling at launch, as a chain's
         rec[i].llr_gain = m.llr_gain;
     }
     const size_t mb = (size_t)d_micro_batch;
@@ -292,7 +358,16 @@ void rx_fec_stream_pool::launch_slot(flight_slot& s)
         hipStream_t st = r.stream;
         hip_check(hipMemcpyAsync(r.sym_dev, s.syms, (size_t)s.n_syms * sizeof(complexf), hipMemcpyHostToDevice, st),
                   "copy in");
-        abi_check(cusat_stream_rx_plan_async(r.sc, rec, n, st), "plan");
+        abi_check(cusat_stream_rx_plan_async(r.sc, rec, n, s.events.data(), (int)s.events.size(), st), "plan");
+        if (d_gpu_dd) {
+            // the SNR state passes from micro-batch to micro-batch in launch order
+            if (d_dd_recorded) hip_check(hipStreamWaitEvent(st, d_dd_done, 0), "wait");
+            hip_check(hipGraphLaunch(r.graph_dd, st), "graph launch");
+            hip_check(hipEventRecord(d_dd_done, st), "event");
+            d_dd_recorded = true;
+            hip_check(hipMemcpyAsync(s.state, r.state_dev, d_mc.size() * 4 * sizeof(float), hipMemcpyDeviceToHost, st),
+                      "copy out");
+        }
         hip_check(hipGraphLaunch(r.graph, st), "graph launch");
         hip_check(hipMemcpyAsync(s.res, r.res_dev, 3 * mb * sizeof(int32_t), hipMemcpyDeviceToHost, st), "copy out");
         if (s.n_out > 0)
@@ -308,6 +383,15 @@ void rx_fec_stream_pool::post_process(flight_slot& s, block*& cur)
     const int32_t* trials = s.res;
     const int32_t* valid = s.res + mb;
     const int32_t* dfl = s.res + 2 * mb;
+    if (d_gpu_dd)
+        for (auto& m : d_mc) {
+            // the state after this micro-batch, unless an update_snr came later
+            if (s.batch < m.host_event_batch) continue;
+            const float* st = s.state + 4 * m.code;
+            m.noise_var = st[0];
+            m.llr_gain = st[1];
+            m.snr_db = std::isnan(st[3]) ? 10.0f * std::log10(st[2]) : st[3];
+        }
     for (int i = 0; i < s.n_frames; i++) {
         modcod_state& m = d_mc[s.mc[i]];
         m.frames++;
@@ -329,11 +413,12 @@ void rx_fec_stream_pool::post_process(flight_slot& s, block*& cur)
     }
 }
 
-bool rx_fec_stream_pool::drain(bool wait_all)
+bool rx_fec_stream_pool::drain(bool wait_all, bool oldest_only)
 {
     bool any = false;
     block* cur = nullptr;  // consecutive data fields of one MODCOD share a block
     for (;;) {
+        if (oldest_only && any) break;
         if (wait_all && d_drain_idx >= d_launch_idx && d_drain_idx < d_submit_idx)
             pump_launches(true);  // the oldest micro-batch may still wait for budget
         if (d_drain_idx >= d_launch_idx) break;
@@ -414,6 +499,7 @@ void rx_fec_stream_pool::reset_all()
         m.llr_gain = 4.0f;
         m.frames = m.errors = 0;
     }
+    reset_state();  // every launch has completed
 }
 
 void rx_fec_stream_pool::warmup_all()

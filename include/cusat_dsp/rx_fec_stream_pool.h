@@ -20,9 +20,16 @@
  *    is decoded bit-identically to its MODCOD's chain at the same scaling);
  *  - an output block holds one MODCOD's consecutive data fields of one drain
  *    (block boundaries may fall elsewhere than rx_fec_pool's);
- *  - the demapper scaling of a frame is its MODCOD's state when the
- *    micro-batch is launched (a chain's likewise), with the decision-directed
- *    SNR refreshed every 32nd frame of that MODCOD (rx_fec_chain.cc:321-342);
+ *  - the demapper scaling of a frame is its MODCOD's SNR state when the
+ *    micro-batch is closed (a chain reads it when it launches one; the two
+ *    agree whenever the budget lets a micro-batch launch at once).  The
+ *    decision-directed SNR is refreshed every 32nd frame of that MODCOD
+ *    (rx_fec_chain.cc:321-342).  By default the estimate runs on the GPU in
+ *    the micro-batch's own launch sequence (cusat_stream_rx_dd: bit-identical
+ *    sums, no host pass over the symbols); update_snr values travel with the
+ *    micro-batch as events, in submit order; data_snr_db() follows once a
+ *    micro-batch is drained.  set_gpu_dd_snr(false) estimates on the host at
+ *    submit instead, as a chain does;
  *  - micro_batch = frames per launch across all MODCODs; n_gpu_streams
  *    streams (each with its own stream context and captured graph) overlap
  *    copies and launches; the decode budget caps frames in flight as
@@ -43,6 +50,7 @@
 #include <vector>
 
 #include "acm_config.h"
+#include "cusat_fec.h"
 #include "plframe.h"
 #include "rx_fec_chain.h"
 
@@ -95,6 +103,12 @@ public:
     uint64_t total_gpu_drop_count() const { return d_gpu_drop_cnt; }
     float data_snr_db() const;
 
+    /** true (default): decision-directed SNR on the GPU; false: on the host.
+     *  Set before the first submit. */
+    void set_gpu_dd_snr(bool on) { d_gpu_dd = on; }
+    /** Wait for the oldest micro-batch instead of dropping a frame when the
+     *  ring is full (rx_fec_chain::set_block_when_full). */
+    void set_block_when_full(bool on) { d_block_when_full = on; }
     int micro_batch() const { return d_micro_batch; }
     int pending_count() const { return d_batch_idx; }
 
@@ -109,17 +123,21 @@ private:
         std::vector<complexf> points;
         float snr_db = 0, pilot_snr_db = 0, noise_var = 1.0f, llr_gain = 4.0f;
         int snr_skip = 0;
+        int host_event_batch = -1;  // micro-batch that carries the latest update_snr
         uint64_t frames = 0, errors = 0;
     };
     struct flight_slot {
         complexf* syms = nullptr;  // pinned, packed frames
         int32_t* res = nullptr;    // pinned: trials | valid | dfl, micro_batch each
         uint8_t* out = nullptr;    // pinned data fields, packed
+        float* state = nullptr;    // pinned: SNR state per MODCOD after the micro-batch
         ihipEvent_t* event = nullptr;
+        std::vector<cusat_stream_rx_event> events;  // SNR events, submit order
         std::vector<int> mc;       // modcod_state index per frame
         std::vector<int64_t> sym_off, out_off;
         int64_t n_syms = 0, n_out = 0;
         int n_frames = 0;
+        int batch = 0;             // micro-batch number (submit order)
         bool in_flight = false;
     };
     struct stream_res {
@@ -128,7 +146,9 @@ private:
         complexf* sym_dev = nullptr;
         uint8_t* out_dev = nullptr;
         int32_t* res_dev = nullptr;
-        hipGraphExec* graph = nullptr;
+        float* state_dev = nullptr;      // state after the stream's latest micro-batch
+        hipGraphExec* graph = nullptr;   // cusat_stream_rx
+        hipGraphExec* graph_dd = nullptr;  // cusat_stream_rx_dd
     };
     struct block {
         std::vector<uint8_t> bytes;
@@ -151,12 +171,20 @@ private:
     std::chrono::steady_clock::time_point d_batch_start_time;
     std::mutex* d_mtx = nullptr;
     std::shared_ptr<gpu_decode_budget> d_budget;
+    bool d_gpu_dd = true;
+    bool d_block_when_full = false;
+    float* d_state = nullptr;                 // device SNR state, float4 per MODCOD
+    ihipEvent_t* d_dd_done = nullptr;         // recorded after the latest SNR launch
+    bool d_dd_recorded = false;
+    std::vector<cusat_stream_rx_event> d_events;  // events of the micro-batch being filled
 
     void update_dd_snr(modcod_state& m, const complexf* frame);
+    void add_host_event(const modcod_state& m, float snr_lin, float snr_db);
+    void reset_state();
     void close_batch();
     void pump_launches(bool bypass_cap = false);
     void launch_slot(flight_slot& s);
-    bool drain(bool wait_all);
+    bool drain(bool wait_all, bool oldest_only = false);
     void post_process(flight_slot& s, block*& cur);
     void release_budget(int n)
     {

@@ -260,21 +260,50 @@ int cusat_stream_get_tuning(const cusat_stream_ctx* sc, cusat_stream_tuning* out
  * cusat_ldpc_decode + cusat_bch_check + cusat_bb_extract of the frame's own
  * context.  The three launches read the batch from device memory, so a
  * captured cusat_stream_rx graph replays any plan. */
+#define CUSAT_RX_SCALE_STATE 0x1  /* demap with the code's SNR state after the batch's events */
 typedef struct cusat_stream_rx_frame {
     int32_t code;          /* index into the contexts of cusat_stream_create */
-    int32_t reserved;
+    int32_t flags;         /* 0 or CUSAT_RX_SCALE_STATE */
     int64_t sym_offset;    /* first symbol (complex float pair) of the XFECFRAME in sym */
     int64_t out_offset;    /* first byte of the frame's data-field slot (k_bch/8 - 10 bytes) in out */
-    float noise_var;       /* demapper scaling, as cusat_fec_demap takes it */
+    float noise_var;       /* demapper scaling, as cusat_fec_demap takes it (flags 0) */
     float llr_gain;
 } cusat_stream_rx_frame;
+/* ---- LLR-scale front door: decision-directed SNR on the GPU ----
+ * Replaces the host passes of rx_fec_chain::update_dd_snr / update_snr
+ * (rx_fec_chain.cc:321-342, modulation/dd_snr.cc:23-51).  A per-code SNR
+ * state lives in device memory (float4 per code: noise_var, llr_gain,
+ * snr_lin, snr_db; the caller allocates and initialises it, e.g. 1, 4, 0, 0,
+ * and shares it between the stream contexts of one receiver).  A batch's
+ * events apply to it in order: frame >= 0 = the decision-directed estimate
+ * of that frame of the batch (QPSK: every symbol, sign decisions; other
+ * constellations: every 8th symbol, nearest point; float sums in symbol
+ * order, bit-identical to the host estimate), applied when above 1;
+ * frame = -1 = the value snr_lin (an update_snr, snr_db kept for
+ * reporting).  noise_var = 1 / snr_lin, llr_gain = min(4 / noise_var, 1000).
+ * Frames flagged CUSAT_RX_SCALE_STATE are demapped with their code's state
+ * after all the batch's events (the scaling a chain reads when it launches
+ * a micro-batch). */
+typedef struct cusat_stream_rx_event {
+    int32_t code;
+    int32_t frame;         /* frame of the batch, or -1 */
+    float snr_lin;         /* frame -1: the value */
+    float snr_db;          /* frame -1: reported as is */
+} cusat_stream_rx_event;
 /* Make the micro-batch current (current tuning; staged and copied on
- * `stream` like cusat_stream_plan_async).  LLRs and hard decisions live in
+ * `stream` like cusat_stream_plan_async), with n_events SNR events (at most
+ * 2 max_frames + the number of codes).  LLRs and hard decisions live in
  * work buffers of the stream context (max_frames x the largest code,
  * allocated by the first call).  A later cusat_stream_plan /
  * cusat_stream_plan_async replaces it. */
 int cusat_stream_rx_plan_async(cusat_stream_ctx* sc, const cusat_stream_rx_frame* frames, int n_frames,
-                               void* stream);
+                               const cusat_stream_rx_event* events, int n_events, void* stream);
+/* Apply the current batch's events to state (device, float4 per code) and
+ * snapshot every code's demapper scales for cusat_stream_rx; state_out
+ * (device or pinned, may be NULL) receives a copy of the new state, with
+ * snr_db NaN where an estimate set it.  Launches that share one state must
+ * be ordered (the same stream, or events between streams).  Capturable. */
+int cusat_stream_rx_dd(cusat_stream_ctx* sc, const float* sym, float* state, float* state_out, void* stream);
 /* Run the current rx plan: sym = device symbols, out = device data-field
  * slots, trials / valid / dfl = B ints each by frame number (dfl as
  * cusat_bb_extract writes it: -1 FEC failure, 0 no data field, n bytes). */

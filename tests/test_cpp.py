@@ -38,3 +38,8 @@ def test_cpp_rx_fec_pool_on_gpu():
 @pytest.mark.gpu
 def test_cpp_rx_fec_stream_pool_on_gpu():
     _run("qa_rx_fec_stream_pool_gpu", 110)
+
+
+@pytest.mark.gpu
+def test_cpp_dd_snr_on_gpu():
+    _run("qa_dd_snr_gpu", 110)
