@@ -29,6 +29,7 @@ namespace cusat {
 namespace ldpc_dev {
 
 constexpr int SPLIT_LANE_THREADS = 2 * LANE_THREADS;  // 6 waves per lane
+constexpr int SPLIT_LDS_STATIC = 160 * 1024;           // the whole LDS of a gfx950 workgroup
 
 // message record of one half: (CMAX + 1) / 2 data slots + 1 parity slot
 template <int CMAX>
@@ -463,7 +464,20 @@ __global__ void __launch_bounds__(LANES* SPLIT_LANE_THREADS)
                       const int* __restrict__ beta, const int* __restrict__ alpha,
                       const int32_t* __restrict__ active, kparams p)
 {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
+    // Normal frames (two layers per step) fill most of the LDS, one
+    // workgroup per CU either way: their instantiations take the whole LDS
+    // as a static array, whose address (0) is known when the kernel is
+    // compiled, so no instruction adds the base of a dynamic allocation to
+    // every LDS address.  Short frames keep the dynamic size that lets
+    // several workgroups share a CU.
+    unsigned char* lds_raw;
+    if constexpr (LANES == 2) {
+        __shared__ __align__(16) unsigned char lds_fixed[SPLIT_LDS_STATIC];
+        lds_raw = lds_fixed;
+    } else {
+        extern __shared__ __align__(16) unsigned char lds_dyn[];
+        lds_raw = lds_dyn;
+    }
     const int cw = blockIdx.x;
     if (active && active[cw] == 0) return;
     const int res = decode_frame_split<CMAX, EXACT, LANES>(
@@ -479,7 +493,11 @@ template <int CMAX, bool EXACT, int LANES>
 hipError_t launch_ldpc_split_variant(const ldpc_args& a, int batch, hipStream_t stream)
 {
     auto kern = &ldpc_dev::ldpc_kernel_split<CMAX, EXACT, LANES>;
-    const int lds = ldpc_lds_bytes_split(a.K, a.R, a.n_slots, a.n_steps, a.n_groups, LANES);
+    int lds = ldpc_lds_bytes_split(a.K, a.R, a.n_slots, a.n_steps, a.n_groups, LANES);
+    if (LANES == 2) {  // static LDS (see ldpc_kernel_split)
+        if (lds > ldpc_dev::SPLIT_LDS_STATIC) return hipErrorInvalidValue;
+        lds = 0;
+    }
     static int configured = 0;
     if (lds > __atomic_load_n(&configured, __ATOMIC_ACQUIRE)) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
