@@ -87,3 +87,30 @@ def gpu_bch(ctx, hard):
 def codeword_llr(cw_bytes, n, mag=127):
     bits = np.unpackbits(cw_bytes, axis=-1)[..., :n]
     return np.where(bits == 0, mag, -mag).astype(np.int16)
+
+
+def dd_snr_host(sym, points, qpsk):
+    """Decision-directed SNR as the host computes it (float32, symbol order)."""
+    f32 = np.float32
+    s = f32(0.0)
+    e = f32(0.0)
+    if qpsk:
+        a = f32(0.70710678118654752)
+        for re, im in sym.astype(np.float32):
+            dr = -a if re < 0 else a
+            di = -a if im < 0 else a
+            s = f32(s + f32(f32(dr * dr) + f32(di * di)))
+            e = f32(e + f32(f32(re - dr) * f32(re - dr)))
+            e = f32(e + f32(f32(im - di) * f32(im - di)))
+    else:
+        pts = points.astype(np.float32)
+        for re, im in sym[::8].astype(np.float32):
+            dx = (re - pts[:, 0]).astype(np.float32)
+            dy = (im - pts[:, 1]).astype(np.float32)
+            d = (dx * dx + dy * dy).astype(np.float32)
+            k = int(np.argmin(d))  # first minimum, as the host's strict < scan
+            s = f32(s + f32(f32(pts[k, 0] * pts[k, 0]) + f32(pts[k, 1] * pts[k, 1])))
+            e = f32(e + d[k])
+    if not (np.isfinite(s) and np.isfinite(e)):
+        return f32(0.0)
+    return f32(1e6) if f32(e * f32(1e6)) <= s else f32(s / e)

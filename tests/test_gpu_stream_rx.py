@@ -15,7 +15,7 @@ import pytest
 
 import cusat_amd as ca
 from tests import bb_ref
-from tests.fec_util import snr_scales
+from tests.fec_util import dd_snr_host as _dd_host, snr_scales
 
 pytestmark = pytest.mark.gpu
 
@@ -190,3 +190,49 @@ def test_stream_rx_rejects_bad_records():
         sc.rx_plan_async(recs)
     with pytest.raises(RuntimeError):
         sc.rx_plan_async(np.zeros(3, ca.STREAM_RX_FRAME))  # more than max_frames
+
+
+def test_stream_rx_scaling_from_the_device_snr_state():
+    """Frames flagged RX_SCALE_STATE demap with their code's SNR state after
+    the batch's events: a host value, or the GPU estimate (checked against a
+    float32 restatement of the host estimate), applied in order."""
+    torch = _torch()
+    mix = [(0, "3/4", "8PSK", 8.6), (1, "1/2", "QPSK", 2.6), (1, "5/6", "16APSK", 11.8)]
+    ctxs, frames = make_batch(mix, 3, 31)
+    n = len(frames)
+    sc = ca.StreamContext(ctxs, n)
+    recs, d_sym, ob = pack(ctxs, frames, np.random.default_rng(8))
+    recs["flags"] = ca.RX_SCALE_STATE
+    recs["noise_var"] = 0.0  # ignored for flagged frames
+    # code 0: host value; code 1: estimate from its first frame; code 2: an
+    # estimate followed by a host value (the host value wins)
+    first = {k: next(i for i, f in enumerate(frames) if f[0] == k) for k in range(3)}
+    v0 = np.float32(10.0 ** 0.86)
+    v2 = np.float32(10.0 ** 1.18)
+    ev = np.zeros(4, ca.STREAM_RX_EVENT)
+    ev[0] = (0, -1, v0, 8.6)
+    ev[1] = (1, first[1], 0.0, 0.0)
+    ev[2] = (2, first[2], 0.0, 0.0)
+    ev[3] = (2, -1, v2, 11.8)
+    state = torch.tensor([[1.0, 4.0, 0.0, 0.0]] * 3, dtype=torch.float32, device="cuda")
+    state_out = torch.zeros_like(state)
+    out, tr, va, dfl = outputs(n, ob)
+    sc.rx_plan_async(recs, ev)
+    sc.rx_dd(d_sym, state, state_out)
+    sc.rx(d_sym, out, tr, va, dfl)
+    torch.cuda.synchronize()
+    st = state_out.cpu().numpy()
+    est1 = _dd_host(frames[first[1]][1], None, True)  # QPSK: sign decisions
+    want_lin = [v0, est1 if est1 > 1 else np.float32(0.0), v2]
+    for k in range(3):
+        assert st[k, 2].view(np.uint32) == np.float32(want_lin[k]).view(np.uint32), (k, st[k], want_lin[k])
+        nv = np.float32(np.float32(1.0) / np.float32(want_lin[k]))
+        assert st[k, 0].view(np.uint32) == nv.view(np.uint32)
+        assert st[k, 1] == min(np.float32(np.float32(4.0) / nv), np.float32(1000.0))
+    assert np.isnan(st[1, 3]) and st[0, 3] == np.float32(8.6) and st[2, 3] == np.float32(11.8)
+    # the frames were demapped with these states: per-frame references at
+    # the same (noise_var, llr_gain) give the same outputs
+    ref = reference(ctxs, [(k, sym, float(st[k, 0]), float(st[k, 1]), pay) for k, sym, _, _, pay in frames])
+    check(frames, recs, ref, out, tr, va, dfl)
+    # the device state itself was updated for the next batch
+    assert np.array_equal(state.cpu().numpy(), st, equal_nan=True)

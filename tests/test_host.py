@@ -264,3 +264,30 @@ def test_encode_frames_is_gen_frames_chain():
     s2, cw2 = ca.encode_frames(0, "2/3", "16APSK", 9, info, 1000.0, want_cw=True)
     np.testing.assert_array_equal(cw1, cw2)
     np.testing.assert_array_equal(s1, s2)
+
+
+def test_dd_snr_restatement_equals_the_library_host_estimate():
+    """The float32 restatement of the decision-directed SNR the GPU tests
+    check against (tests/fec_util.dd_snr_host) equals libcusat_dsp's host
+    estimate (qpsk_snr_dd / constellation_snr_dd) bit for bit."""
+    from tests.fec_util import dd_snr_host
+    L = ctypes.CDLL(os.path.join(os.path.dirname(ca.__file__), "libcusat_dsp.so"))
+    q = L._ZN9cusat_dsp11qpsk_snr_ddEPKSt7complexIfEi
+    q.restype, q.argtypes = ctypes.c_float, [ctypes.c_void_p, ctypes.c_int]
+    c = L._ZN9cusat_dsp20constellation_snr_ddEPKSt7complexIfEiS3_ii
+    c.restype = ctypes.c_float
+    c.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    rng = np.random.default_rng(4)
+    pts16 = ca.modcod_constellation(0, "2/3", "16APSK")[0].reshape(-1, 2).astype(np.float32)
+    for sigma in (0.05, 0.4, 2.0):
+        sym = (rng.choice([-0.70710677, 0.70710677], (3000, 2)) + rng.normal(0, sigma, (3000, 2))).astype(np.float32)
+        got = q(sym.ctypes.data, len(sym))
+        assert np.float32(got).view(np.uint32) == dd_snr_host(sym, None, True).view(np.uint32)
+        sym = (pts16[rng.integers(0, 16, 4001)] + rng.normal(0, sigma, (4001, 2))).astype(np.float32)
+        got = c(pts16.ctypes.data, 16, sym.ctypes.data, len(sym), 8)
+        assert np.float32(got).view(np.uint32) == dd_snr_host(sym, pts16, False).view(np.uint32)
+    zero = np.zeros((64, 2), np.float32)
+    assert q(zero.ctypes.data, 64) == dd_snr_host(zero, None, True)
+    bad = zero.copy()
+    bad[8, 0] = np.nan
+    assert c(pts16.ctypes.data, 16, bad.ctypes.data, 64, 8) == dd_snr_host(bad, pts16, False) == 0.0
