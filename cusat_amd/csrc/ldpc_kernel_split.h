@@ -383,18 +383,34 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                     return (mag ^ sgn) - sgn;
                 };
                 s2 ys[NEH];
+                if (slotu == 0) {
+                    // no edge of this quarter collides (most quarters): every
+                    // edge updates the APP, no per-edge scalar test
 #pragma unroll
-                for (int e = 0; e < NE0; e++) {
-                    ys[e] = zero;
-                    if (e < ne) {
-                        const s2 y = out(mv[e], xv[e]);
-                        ys[e] = y;
-                        const s2 d = y - ov[e];
-                        if ((slotu >> e) & 1u) {
-                            const circ_rec c = cc[e];
-                            lds_st(lds_raw, ra_[e] + (c.slot_off - c.app_off), perm(bits(d), sels[e]));
-                        } else {
-                            lds_st(lds_raw, ra_[e], perm(bits(__builtin_elementwise_add_sat(av[e], d)), sels[e]));
+                    for (int e = 0; e < NE0; e++) {
+                        ys[e] = zero;
+                        if (e < ne) {
+                            const s2 y = out(mv[e], xv[e]);
+                            ys[e] = y;
+                            lds_st(lds_raw, ra_[e],
+                                   perm(bits(__builtin_elementwise_add_sat(av[e], y - ov[e])), sels[e]));
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int e = 0; e < NE0; e++) {
+                        ys[e] = zero;
+                        if (e < ne) {
+                            const s2 y = out(mv[e], xv[e]);
+                            ys[e] = y;
+                            const s2 d = y - ov[e];
+                            if ((slotu >> e) & 1u) {
+                                const circ_rec c = cc[e];
+                                lds_st(lds_raw, ra_[e] + (c.slot_off - c.app_off), perm(bits(d), sels[e]));
+                            } else {
+                                lds_st(lds_raw, ra_[e],
+                                       perm(bits(__builtin_elementwise_add_sat(av[e], d)), sels[e]));
+                            }
                         }
                     }
                 }
