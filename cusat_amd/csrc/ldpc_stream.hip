@@ -195,9 +195,11 @@ __device__ __forceinline__ int bch_frame(unsigned char* lds, int* flag, const ui
     return ok;
 }
 
-// ctl_off: byte offset in the dynamic LDS of the two control ints (queue
-// slot, BCH flag), after the decoders' own LDS
-template <bool WIDE>
+// ctl_off: byte offset in the LDS of the two control ints (queue slot, BCH
+// flag), after the decoders' own LDS.  FIXED: the whole LDS as a static
+// array (address known at compile time, no base add on every LDS access),
+// for streams whose LDS allows one workgroup per CU anyway
+template <bool WIDE, bool FIXED = false>
 __global__ void __launch_bounds__(stream_threads<WIDE>())
     ldpc_stream_kernel(const code_desc* __restrict__ codes, const cusat_stream_frame* __restrict__ frames,
                        const int2* __restrict__ jobs, const int* __restrict__ order,
@@ -206,7 +208,14 @@ __global__ void __launch_bounds__(stream_threads<WIDE>())
                        int32_t* __restrict__ fallback, uint4* __restrict__ ws, size_t ws_v4, int learned,
                        int ctl_off)
 {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
+    unsigned char* lds_raw;
+    if constexpr (FIXED) {
+        __shared__ __align__(16) unsigned char lds_fixed[ldpc_dev::SPLIT_LDS_STATIC];
+        lds_raw = lds_fixed;
+    } else {
+        extern __shared__ __align__(16) unsigned char lds_dyn[];
+        lds_raw = lds_dyn;
+    }
     int& job_s = reinterpret_cast<int*>(lds_raw + ctl_off)[0];
     int& flag_s = reinterpret_cast<int*>(lds_raw + ctl_off)[1];
     uint4* rec = ws + (size_t)blockIdx.x * ws_v4;
@@ -444,6 +453,11 @@ hipError_t fit_kernel(int lds, int* occ)
     static int configured = 0;
     auto kern = &ldpc_stream_kernel<WIDE>;
     *occ = 0;
+    if (WIDE && lds > ldpc_dev::SPLIT_LDS_STATIC / 2 && lds <= ldpc_dev::SPLIT_LDS_STATIC) {
+        // one workgroup per CU either way: the static-LDS build
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, &ldpc_stream_kernel<true, true>,
+                                                            stream_threads<true>(), 0);
+    }
     if (lds > __atomic_load_n(&configured, __ATOMIC_ACQUIRE)) {
         const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, lds);
@@ -673,7 +687,12 @@ hipError_t launch_stream(cusat_stream_ctx* sc, const int16_t* llr, uint8_t* hard
     const int grid = sc->n_cu * sc->tuning.wg_per_cu;
     const int w = sc->tuning.split_edges;
     const int learned = (sc->flags & CUSAT_STREAM_LEARNED) ? 1 : 0;
-    if (w)
+    if (w && sc->lds[1] > ldpc_dev::SPLIT_LDS_STATIC / 2)  // as fit_kernel chose
+        hipLaunchKernelGGL((ldpc_stream_kernel<true, true>), dim3(grid), dim3(stream_threads<true>()), 0,
+                           (hipStream_t)stream, sc->d_codes, sc->d_frames, sc->d_jobs, sc->d_order,
+                           sc->d_counters, llr, hard, trials, valid, fallback, sc->d_ws, sc->ws_v4, learned,
+                           sc->lds[1] - 16);
+    else if (w)
         hipLaunchKernelGGL(ldpc_stream_kernel<true>, dim3(grid), dim3(stream_threads<true>()), sc->lds[1],
                            (hipStream_t)stream, sc->d_codes, sc->d_frames, sc->d_jobs, sc->d_order,
                            sc->d_counters, llr, hard, trials, valid, fallback, sc->d_ws, sc->ws_v4, learned,
