@@ -73,7 +73,10 @@ size_t ldpc_msg_bytes_split(int q, int cmax)
 
 hipError_t launch_ldpc_decode(const ldpc_args& a, int batch, hipStream_t stream)
 {
-    if (a.split && ldpc_split_available(a.cmax, a.exact, a.lanes)) {
+    // the split kernel only where its LDS fits a workgroup (the 6-wave
+    // kernel needs less: no partial-exchange area)
+    if (a.split && ldpc_split_available(a.cmax, a.exact, a.lanes) &&
+        ldpc_lds_bytes_split(a.K, a.R, a.n_slots, a.n_steps, a.n_groups, a.lanes) <= 160 * 1024) {
 #define SCASE(C, E, L) \
     if (a.cmax == C && a.exact == E && a.lanes == L) return launch_ldpc_split_variant<C, E, L>(a, batch, stream);
         SCASE(2, true, 2) SCASE(3, true, 2) SCASE(4, true, 2) SCASE(5, true, 2) SCASE(7, true, 2)
