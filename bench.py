@@ -15,6 +15,16 @@ LDPC kernel's roofline (algorithmic HBM bytes per launch over its measured
 average duration, HIP events on the launch stream) and the CPU baseline
 (the C restatement of the reference decoder, multithreaded, on a bounded
 sample of the same frames).
+
+The same line carries "learned_vs_plain", measured in the same run: the
+learned offset / normalised min-sum decoder with plain fallback against
+the plain decoder at their FER-1e-3 operating point.  The workload is
+DVB-S2 normal 16APSK 2/3, 16384 fresh frames from the GPU generator at
+Es/N0 9.2 dB, and each decoder is one persistent launch over all of them
+(decode + BCH, the learned decoder re-decoding failing frames with the
+plain one in the same launch).  Per decoder the object reports decoded
+info Gbit/s, FER (frames whose info bits differ from the sent ones) and
+mean iterations; every frame is also checked against the batched decoders.
 """
 import argparse
 import json
@@ -43,6 +53,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", type=int, default=8, help="frames checked against the oracle")
     ap.add_argument("--no-split", action="store_true", help="decode with the 6-wave kernel")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain measurement")
+    ap.add_argument("--learned-frames", type=int, default=16384)
+    ap.add_argument("--learned-esn0", type=float, default=9.2)
     return ap.parse_args()
 
 
@@ -171,7 +184,28 @@ def main():
                      "traffic": traffic},
         "cpu_baseline": cpu,
     }
+    if not args.no_learned:
+        out["learned_vs_plain"] = learned_vs_plain(args.learned_frames, args.learned_esn0)
     print(json.dumps(out))
+
+
+def learned_vs_plain(frames, es):
+    """The learned decoder (trained parameters of DVB-S2 normal 16APSK 2/3)
+    with plain fallback against the plain decoder, each as one persistent
+    launch over `frames` fresh GPU frames at Es/N0 `es`."""
+    from cusat_amd import learn
+    rows = learn.evaluate_stream(points=(es,), frames=frames, source="gpu", reps=3, log=lambda s: None)
+    r = {x["decoder"].split(" ")[0]: x for x in rows}
+    keep = ("gbps_info", "fer", "mean_iterations", "ms_launch")
+    res = {"workload": "DVB-S2 normal 16APSK 2/3, fresh GPU-generated frames, one persistent launch each",
+           "es_n0_db": es, "frames": frames,
+           "plain": {k: round(r["plain"][k], 6) for k in keep},
+           "learned_fallback": {k: round(r["learned+fallback"][k], 6) for k in keep},
+           "identical_to_batched": bool(r["plain"]["identical_to_batched"] and
+                                        r["learned+fallback"]["identical_to_batched"])}
+    res["learned_fallback"]["fallback_rate"] = round(r["learned+fallback"]["fallback_rate"], 6)
+    res["speedup"] = round(res["learned_fallback"]["gbps_info"] / res["plain"]["gbps_info"], 4)
+    return res
 
 
 if __name__ == "__main__":
