@@ -9,10 +9,13 @@
 //    of the frames that pass come out in arrival order;
 //  - launch_stale sends a partial micro-batch and drain_all_completed
 //    collects it; reset_all drops queued output; warmup_all leaves clean
-//    counters; an unknown MODCOD index in the constructor throws.
+//    counters; a full ring drops frames (counted as errors) while the rest
+//    still leave in arrival order, or blocks when set_block_when_full is on;
+//    an unknown MODCOD index in the constructor throws.
 // Fewer than 32 frames of each MODCOD are sent, so the decision-directed SNR
 // re-estimate (every 32nd frame of a MODCOD) never fires and both pools scale
 // every frame by the update_snr value.
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -241,6 +244,55 @@ int main()
              std::vector<uint8_t> want;
              for (const auto& p : t.payloads) want.insert(want.end(), p.begin(), p.end());
              QA_CHECK(got.bytes == want);
+         }},
+        {"full_ring_drops_in_order_or_blocks",
+         [] {
+             // micro-batches of one frame: 512 slots fill long before the GPU
+             // has drained them when 1500 frames arrive back to back
+             const modcod_def_t* m = find_modcod(4);
+             traffic t = make_traffic(FECFRAME_SHORT, m->code_rate, m->constellation, 50, 6.0f, 60);
+             const int n = 1500;
+             for (int block = 0; block < 2; block++) {
+                 rx_fec_stream_pool pool(FECFRAME_SHORT, {4}, 50, 64, 2, 1);
+                 pool.set_block_when_full(block == 1);
+                 pool.update_snr(4, 6.0f);
+                 pool_output got;
+                 std::vector<int> sent;
+                 for (int k = 0; k < n; k++) {
+                     sent.push_back(k % 50);
+                     if (pool.submit(&t.symbols[(size_t)(k % 50) * t.xfec], t.xfec, layout_of(4))) got.take_all(pool);
+                 }
+                 pool.finish_all_pending();
+                 got.take_all(pool);
+                 QA_CHECK_EQ(pool.total_frame_count(), (uint64_t)n);
+                 QA_CHECK_EQ(pool.total_error_count(), pool.total_gpu_drop_count());  // only drops fail
+                 if (block) QA_CHECK_EQ(pool.total_gpu_drop_count(), (uint64_t)0);
+                 std::printf("block_when_full %d: %llu of %d frames dropped\n", block,
+                             (unsigned long long)pool.total_gpu_drop_count(), n);
+                 // the output is the payloads of the kept frames, in arrival order
+                 size_t pos = 0, byte = 0;
+                 int kept = 0;
+                 bool in_order = true;
+                 for (int sz : got.sizes) {
+                     auto matches = [&](size_t p) {
+                         const auto& pl = t.payloads[sent[p]];
+                         return (int)pl.size() == sz && std::equal(pl.begin(), pl.end(), got.bytes.begin() + byte);
+                     };
+                     while (pos < sent.size() && !matches(pos)) pos++;
+                     if (pos == sent.size()) {
+                         in_order = false;
+                         break;
+                     }
+                     byte += sz;
+                     pos++;
+                     kept++;
+                 }
+                 QA_CHECK(in_order);
+                 QA_CHECK_EQ(byte, got.bytes.size());
+                 int want_kept = 0;  // non-empty payloads among the frames that were not dropped
+                 for (int k : sent) want_kept += t.payloads[k].empty() ? 0 : 1;
+                 if (block) QA_CHECK_EQ(kept, want_kept);
+             }
          }},
         {"unknown_index_in_constructor_throws",
          [] {
