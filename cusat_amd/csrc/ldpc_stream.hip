@@ -684,7 +684,10 @@ namespace {
 hipError_t launch_stream(cusat_stream_ctx* sc, const int16_t* llr, uint8_t* hard, int32_t* trials, int32_t* valid,
                          int32_t* fallback, void* stream)
 {
-    const int grid = sc->n_cu * sc->tuning.wg_per_cu;
+    // never more workgroups than a plan can have jobs: a context for small
+    // batches (an ACM receiver's micro-batches) leaves the other CUs to the
+    // next batch's launches instead of parking idle workgroups on them
+    const int grid = std::min(sc->n_cu * sc->tuning.wg_per_cu, std::max(1, sc->max_frames));
     const int w = sc->tuning.split_edges;
     const int learned = (sc->flags & CUSAT_STREAM_LEARNED) ? 1 : 0;
     if (w && sc->lds[1] > ldpc_dev::SPLIT_LDS_STATIC / 2)  // as fit_kernel chose
@@ -798,12 +801,20 @@ extern "C" int cusat_stream_rx_plan_async(cusat_stream_ctx* sc, const cusat_stre
     sc->n_frames = n_frames;
     const int nj = stage_plan(sc, sc->tuning, img);
     hipStream_t st = (hipStream_t)stream;
-    HIP_TRY(copy_plan(sc, img, n_frames, nj, st, true));
-    HIP_TRY(hipMemcpyAsync(reinterpret_cast<unsigned char*>(sc->d_counters) + sc->off_rx, img + sc->off_rx,
-                           256 + n_frames * sizeof(rx_rec), hipMemcpyHostToDevice, st));
-    if (n_events)
-        HIP_TRY(hipMemcpyAsync(reinterpret_cast<unsigned char*>(sc->d_counters) + sc->off_ev, img + sc->off_ev,
-                               n_events * sizeof(rx_event), hipMemcpyHostToDevice, st));
+    unsigned char* d = reinterpret_cast<unsigned char*>(sc->d_counters);
+    const size_t span = sc->off_ev + n_events * sizeof(rx_event);
+    if (span <= (256u << 10)) {
+        // one copy of the whole image up to the last event (the unused
+        // tails of its arrays ride along; no launch reads past a count)
+        HIP_TRY(hipMemcpyAsync(d, img, span, hipMemcpyHostToDevice, st));
+    } else {
+        HIP_TRY(copy_plan(sc, img, n_frames, nj, st, true));
+        HIP_TRY(hipMemcpyAsync(d + sc->off_rx, img + sc->off_rx, 256 + n_frames * sizeof(rx_rec),
+                               hipMemcpyHostToDevice, st));
+        if (n_events)
+            HIP_TRY(hipMemcpyAsync(d + sc->off_ev, img + sc->off_ev, n_events * sizeof(rx_event),
+                                   hipMemcpyHostToDevice, st));
+    }
     HIP_TRY(hipEventRecord(sc->stage_done[k], st));
     sc->stage_used[k] = true;
     sc->next_stage = (k + 1) % cusat_stream_ctx::N_STAGE;

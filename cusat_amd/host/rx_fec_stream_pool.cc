@@ -9,6 +9,8 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <immintrin.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -32,6 +34,40 @@ void abi_check(int rc, const char* what)
     if (rc != CUSAT_OK)
         throw std::runtime_error(std::string("rx_fec_stream_pool: ") + what + ": " + cusat_last_error());
 }
+
+// A frame's symbols into its pinned slot (32-byte aligned).  The slot is
+// written once and then read by the DMA engine, so streaming stores skip
+// the read-for-ownership of every destination line; frames are far below
+// the size at which memcpy switches to such stores itself.
+__attribute__((target("avx2"))) void copy_stream_avx2(void* dst, const void* src, size_t bytes)
+{
+    char* d = static_cast<char*>(dst);
+    const char* s = static_cast<const char*>(src);
+    size_t i = 0;
+    for (; i + 128 <= bytes; i += 128) {
+        const __m256i a = _mm256_loadu_si256(reinterpret_cast<const __m256i*>(s + i));
+        const __m256i b = _mm256_loadu_si256(reinterpret_cast<const __m256i*>(s + i + 32));
+        const __m256i c = _mm256_loadu_si256(reinterpret_cast<const __m256i*>(s + i + 64));
+        const __m256i e = _mm256_loadu_si256(reinterpret_cast<const __m256i*>(s + i + 96));
+        _mm256_stream_si256(reinterpret_cast<__m256i*>(d + i), a);
+        _mm256_stream_si256(reinterpret_cast<__m256i*>(d + i + 32), b);
+        _mm256_stream_si256(reinterpret_cast<__m256i*>(d + i + 64), c);
+        _mm256_stream_si256(reinterpret_cast<__m256i*>(d + i + 96), e);
+    }
+    _mm_sfence();  // the streaming stores are visible before the slot is handed to the GPU
+    if (i < bytes) memcpy(d + i, s + i, bytes - i);
+}
+
+void copy_frame(void* dst, const void* src, size_t bytes)
+{
+    static const bool avx2 = __builtin_cpu_supports("avx2");
+    if (avx2)
+        copy_stream_avx2(dst, src, bytes);
+    else
+        memcpy(dst, src, bytes);
+}
+
+size_t round256(size_t n) { return (n + 255) & ~(size_t)255; }
 
 }  // namespace
 
@@ -78,6 +114,8 @@ rx_fec_stream_pool::rx_fec_stream_pool(fecframe_size_t framesize, const std::vec
         }
         const size_t mb = (size_t)micro_batch;
         const size_t state_bytes = std::max<size_t>(d_mc.size(), 1) * 4 * sizeof(float);
+        d_res_bytes = round256(3 * mb * sizeof(int32_t));
+        d_state_bytes = round256(state_bytes);
         hip_check(hipMalloc(&d_state, state_bytes), "state");
         {
             hipEvent_t ev = nullptr;
@@ -92,10 +130,12 @@ rx_fec_stream_pool::rx_fec_stream_pool(fecframe_size_t framesize, const std::vec
             r.stream = st;
             if (d_ctx.empty()) continue;
             abi_check(cusat_stream_create(&r.sc, d_ctx.data(), (int)d_ctx.size(), micro_batch, 0), "stream context");
-            hip_check(hipMalloc(&r.sym_dev, mb * d_max_xfec * sizeof(complexf)), "device buffers");
-            hip_check(hipMalloc(&r.out_dev, mb * d_max_dfl), "device buffers");
-            hip_check(hipMalloc(&r.res_dev, 3 * mb * sizeof(int32_t)), "device buffers");
-            hip_check(hipMalloc(&r.state_dev, state_bytes), "device buffers");
+            hip_check(hipMalloc(&r.sym_dev, mb * (d_max_xfec + 8) * sizeof(complexf)), "device buffers");
+            hip_check(hipMalloc(&r.blk, d_res_bytes + d_state_bytes + std::max<size_t>(mb * d_max_dfl, 1)),
+                      "device buffers");
+            r.res_dev = reinterpret_cast<int32_t*>(r.blk);
+            r.state_dev = reinterpret_cast<float*>(r.blk + d_res_bytes);
+            r.out_dev = r.blk + d_res_bytes + d_state_bytes;
             // capture the SNR launch and the three receive launches once;
             // every micro-batch replays them
             abi_check(cusat_stream_rx_plan_async(r.sc, nullptr, 0, nullptr, 0, st), "plan");
@@ -127,18 +167,17 @@ rx_fec_stream_pool::rx_fec_stream_pool(fecframe_size_t framesize, const std::vec
         // an output block never exceeds the ring's frames of the largest code
         d_max_output_bytes = (d_max_dfl + 10) * d_n_flight * micro_batch;
         for (auto& f : d_flight) {
-            hip_check(hipHostMalloc((void**)&f.syms, mb * d_max_xfec * sizeof(complexf), hipHostMallocDefault),
-                      "pinned slots");
-            hip_check(hipHostMalloc((void**)&f.res, 3 * mb * sizeof(int32_t), hipHostMallocDefault), "pinned slots");
-            hip_check(hipHostMalloc((void**)&f.out, std::max<size_t>(mb * d_max_dfl, 1), hipHostMallocDefault),
-                      "pinned slots");
+            const size_t sym_bytes = mb * (d_max_xfec + 8) * sizeof(complexf);
+            const size_t blk_bytes = d_res_bytes + d_state_bytes + std::max<size_t>(mb * d_max_dfl, 1);
+            hip_check(hipHostMalloc((void**)&f.syms, sym_bytes, hipHostMallocDefault), "pinned slots");
+            hip_check(hipHostMalloc((void**)&f.blk, blk_bytes, hipHostMallocDefault), "pinned slots");
+            f.res = reinterpret_cast<int32_t*>(f.blk);
+            f.state = reinterpret_cast<float*>(f.blk + d_res_bytes);
+            f.out = f.blk + d_res_bytes + d_state_bytes;
             // touch every page now: a first write into pinned memory costs a
             // fault per page, which would land in submit
-            memset(f.syms, 0, mb * d_max_xfec * sizeof(complexf));
-            memset(f.res, 0, 3 * mb * sizeof(int32_t));
-            memset(f.out, 0, std::max<size_t>(mb * d_max_dfl, 1));
-            hip_check(hipHostMalloc((void**)&f.state, state_bytes, hipHostMallocDefault), "pinned slots");
-            memset(f.state, 0, state_bytes);
+            memset(f.syms, 0, sym_bytes);
+            memset(f.blk, 0, blk_bytes);
             f.events.reserve(2 * mb + d_mc.size());
             hipEvent_t ev = nullptr;
             hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "events");
@@ -160,20 +199,16 @@ void rx_fec_stream_pool::free_all()
         if (r.stream) hipStreamSynchronize(r.stream);
     for (auto& f : d_flight) {
         if (f.syms) hipHostFree(f.syms);
-        if (f.res) hipHostFree(f.res);
-        if (f.out) hipHostFree(f.out);
-        if (f.state) hipHostFree(f.state);
+        if (f.blk) hipHostFree(f.blk);
         if (f.event) hipEventDestroy(f.event);
         f = flight_slot{};
     }
     for (auto& r : d_streams) {
         if (r.graph) hipGraphExecDestroy(r.graph);
         if (r.graph_dd) hipGraphExecDestroy(r.graph_dd);
-        if (r.state_dev) hipFree(r.state_dev);
         if (r.sc) cusat_stream_destroy(r.sc);
         if (r.sym_dev) hipFree(r.sym_dev);
-        if (r.out_dev) hipFree(r.out_dev);
-        if (r.res_dev) hipFree(r.res_dev);
+        if (r.blk) hipFree(r.blk);
         if (r.stream) hipStreamDestroy(r.stream);
         r = stream_res{};
     }
@@ -280,8 +315,9 @@ bool rx_fec_stream_pool::submit(const complexf* symbols, int n_symbols, const pl
         s.n_syms = s.n_out = 0;
         d_batch_start_time = std::chrono::steady_clock::now();
     }
+    s.n_syms = (s.n_syms + 7) & ~(int64_t)7;  // 64-byte aligned frame starts
     complexf* dst = s.syms + s.n_syms;
-    memcpy(dst, symbols, (size_t)n_symbols * sizeof(complexf));
+    copy_frame(dst, symbols, (size_t)n_symbols * sizeof(complexf));
     if (++m.snr_skip >= 32) {
         m.snr_skip = 0;
         if (!d_gpu_dd) {
@@ -351,7 +387,6 @@ void rx_fec_stream_pool::launch_slot(flight_slot& s)
         rec[i].noise_var = m.noise_var;  // host mode: the MODCOD's scaling at launch, as a chain's
         rec[i].llr_gain = m.llr_gain;
     }
-    const size_t mb = (size_t)d_micro_batch;
     {
         std::unique_lock<std::mutex> lk;
         if (d_mtx) lk = std::unique_lock<std::mutex>(*d_mtx);
@@ -365,13 +400,12 @@ void rx_fec_stream_pool::launch_slot(flight_slot& s)
             hip_check(hipGraphLaunch(r.graph_dd, st), "graph launch");
             hip_check(hipEventRecord(d_dd_done, st), "event");
             d_dd_recorded = true;
-            hip_check(hipMemcpyAsync(s.state, r.state_dev, d_mc.size() * 4 * sizeof(float), hipMemcpyDeviceToHost, st),
-                      "copy out");
         }
         hip_check(hipGraphLaunch(r.graph, st), "graph launch");
-        hip_check(hipMemcpyAsync(s.res, r.res_dev, 3 * mb * sizeof(int32_t), hipMemcpyDeviceToHost, st), "copy out");
-        if (s.n_out > 0)
-            hip_check(hipMemcpyAsync(s.out, r.out_dev, (size_t)s.n_out, hipMemcpyDeviceToHost, st), "copy out");
+        // results, SNR state and data fields in one copy
+        hip_check(hipMemcpyAsync(s.blk, r.blk, d_res_bytes + d_state_bytes + (size_t)s.n_out, hipMemcpyDeviceToHost,
+                                 st),
+                  "copy out");
         hip_check(hipEventRecord(s.event, st), "event");
     }
     s.in_flight = true;

@@ -127,10 +127,12 @@ private:
         uint64_t frames = 0, errors = 0;
     };
     struct flight_slot {
-        complexf* syms = nullptr;  // pinned, packed frames
-        int32_t* res = nullptr;    // pinned: trials | valid | dfl, micro_batch each
-        uint8_t* out = nullptr;    // pinned data fields, packed
-        float* state = nullptr;    // pinned: SNR state per MODCOD after the micro-batch
+        complexf* syms = nullptr;  // pinned, frames at 64-byte aligned offsets
+        // one pinned block, the layout of stream_res::blk: results, SNR state, data fields
+        unsigned char* blk = nullptr;
+        int32_t* res = nullptr;    // trials | valid | dfl, micro_batch each
+        float* state = nullptr;    // SNR state per MODCOD after the micro-batch
+        uint8_t* out = nullptr;    // data fields, packed
         ihipEvent_t* event = nullptr;
         std::vector<cusat_stream_rx_event> events;  // SNR events, submit order
         std::vector<int> mc;       // modcod_state index per frame
@@ -144,9 +146,11 @@ private:
         ihipStream_t* stream = nullptr;
         cusat_stream_ctx* sc = nullptr;
         complexf* sym_dev = nullptr;
-        uint8_t* out_dev = nullptr;
+        // one device block: results | SNR state | data fields, copied back whole
+        unsigned char* blk = nullptr;
         int32_t* res_dev = nullptr;
         float* state_dev = nullptr;      // state after the stream's latest micro-batch
+        uint8_t* out_dev = nullptr;
         hipGraphExec* graph = nullptr;   // cusat_stream_rx
         hipGraphExec* graph_dd = nullptr;  // cusat_stream_rx_dd
     };
@@ -165,6 +169,7 @@ private:
     std::vector<int> d_last_sizes;
     int d_last_output = -1;                  // d_mc position of the latest output block
     int d_micro_batch = 4, d_n_flight = 0, d_max_xfec = 0, d_max_dfl = 0, d_max_output_bytes = 0;
+    size_t d_res_bytes = 0, d_state_bytes = 0;  // offsets of the state and data fields in a block
     int d_batch_idx = 0, d_submit_idx = 0, d_launch_idx = 0, d_drain_idx = 0, d_next_stream = 0;
     uint64_t d_gpu_drop_cnt = 0;
     bool d_diag[3] = {false, false, false};
