@@ -247,13 +247,20 @@ int main()
          }},
         {"full_ring_drops_in_order_or_blocks",
          [] {
-             // micro-batches of one frame: 512 slots fill long before the GPU
-             // has drained them when 1500 frames arrive back to back
+             // micro-batches of one frame, a ring of 512; the shared decode
+             // budget is held by someone else (in_flight above a cap of 1), so
+             // nothing launches on its own: the ring fills after 510 frames
+             // and the rest are dropped, or, with block-when-full, each
+             // submit waits for the oldest micro-batch (launched regardless)
              const modcod_def_t* m = find_modcod(4);
              traffic t = make_traffic(FECFRAME_SHORT, m->code_rate, m->constellation, 50, 6.0f, 60);
-             const int n = 1500;
+             const int n = 600;
              for (int block = 0; block < 2; block++) {
                  rx_fec_stream_pool pool(FECFRAME_SHORT, {4}, 50, 64, 2, 1);
+                 auto held = std::make_shared<gpu_decode_budget>();
+                 held->cap = 1;
+                 held->in_flight = 1000;
+                 pool.set_decode_budget(held);
                  pool.set_block_when_full(block == 1);
                  pool.update_snr(4, 6.0f);
                  pool_output got;
@@ -266,7 +273,7 @@ int main()
                  got.take_all(pool);
                  QA_CHECK_EQ(pool.total_frame_count(), (uint64_t)n);
                  QA_CHECK_EQ(pool.total_error_count(), pool.total_gpu_drop_count());  // only drops fail
-                 if (block) QA_CHECK_EQ(pool.total_gpu_drop_count(), (uint64_t)0);
+                 QA_CHECK_EQ(pool.total_gpu_drop_count(), (uint64_t)(block ? 0 : n - 510));
                  std::printf("block_when_full %d: %llu of %d frames dropped\n", block,
                              (unsigned long long)pool.total_gpu_drop_count(), n);
                  // the output is the payloads of the kept frames, in arrival order
@@ -289,9 +296,9 @@ int main()
                  }
                  QA_CHECK(in_order);
                  QA_CHECK_EQ(byte, got.bytes.size());
-                 int want_kept = 0;  // non-empty payloads among the frames that were not dropped
-                 for (int k : sent) want_kept += t.payloads[k].empty() ? 0 : 1;
-                 if (block) QA_CHECK_EQ(kept, want_kept);
+                 int want_kept = 0;  // non-empty payloads among the frames that were kept
+                 for (int k = 0; k < (block ? n : 510); k++) want_kept += t.payloads[sent[k]].empty() ? 0 : 1;
+                 QA_CHECK_EQ(kept, want_kept);
              }
          }},
         {"unknown_index_in_constructor_throws",
