@@ -10,8 +10,11 @@
 // output collection and finish_all_pending, from host symbols to GSE bytes
 // (host copy into pinned slots, H2D, demap, LDPC, BCH, BB, D2H included).
 //
-// usage: pool_bench [frames] [run] [pool: chain|stream|stream_hostdd|both] [micro_batch] [streams]
-//                   [modcods,comma-separated]
+// usage: pool_bench [frames] [run] [pool: chain|chain_same|stream|stream_hostdd|both] [micro_batch]
+//                   [streams] [modcods,comma-separated]
+// chain: rx_fec_pool with the reference's defaults (micro-batches of 4, 8
+// streams per MODCOD); chain_same: rx_fec_pool with the given micro_batch
+// and streams per MODCOD.
 // The stream pool waits when its ring is full (set_block_when_full), so
 // its rate is what it sustains; stream_hostdd estimates the SNR on the host.
 // prints one JSON line per pool.
@@ -235,6 +238,10 @@ int main(int argc, char** argv)
         // the reference's defaults: micro-batches of 4 over 8 streams per MODCOD
         rx_fec_pool pool(FECFRAME_NORMAL, ids, 25, 64, 8, 4);
         run(pool, "rx_fec_pool", src, arr, run_len, 4, 8);
+    }
+    if (which == "chain_same") {
+        rx_fec_pool pool(FECFRAME_NORMAL, ids, 25, 64, streams, micro_batch);
+        run(pool, "rx_fec_pool", src, arr, run_len, micro_batch, streams);
     }
     if (which == "stream" || which == "stream_hostdd" || which == "both") {
         rx_fec_stream_pool pool(FECFRAME_NORMAL, ids, 25, 64, streams, micro_batch);
