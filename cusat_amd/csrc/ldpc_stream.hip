@@ -540,7 +540,10 @@ extern "C" int cusat_stream_create(cusat_stream_ctx** out, cusat_fec_ctx* const*
         return CUSAT_E_HIP;
     }
     sc->ws_v4 = rec_max / 16;
-    const size_t n_wg = (size_t)sc->n_cu * std::max(sc->max_wg[0], sc->max_wg[1]);
+    // one message workspace per workgroup the launch can have (the grid is
+    // capped at max_frames)
+    const size_t n_wg = std::min((size_t)sc->n_cu * std::max(sc->max_wg[0], sc->max_wg[1]),
+                                 (size_t)std::max(1, max_frames));
     auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
     sc->off_jobs = 256 + al(sizeof(cusat_stream_frame) * max_frames);
     sc->off_order = sc->off_jobs + al(sizeof(int2) * max_frames);

@@ -1,8 +1,8 @@
 // cusat_dsp::rx_fec_chain over the C ABI.  The flight-ring behaviour
 // follows lib/cusat_dsp/rx_fec_chain.cc (submit / launch / drain order,
 // decode budget, drops when the ring is full, block-when-full, the SNR
-// re-estimate every 32 frames); the device work of a micro-batch is
-// cusat_rx_fec_batch + cusat_bb_extract on the slot's stream.
+// re-estimate every 32 frames); the device work of a micro-batch is one
+// replay of a captured cusat_stream_rx graph on the slot's stream.
 #include "cusat_dsp/rx_fec_chain.h"
 
 #include <hip/hip_runtime_api.h>
@@ -13,6 +13,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "copy_frame.h"
 #include "cusat_fec.h"
 
 namespace cusat_dsp {
@@ -29,6 +30,8 @@ void abi_check(int rc, const char* what)
 {
     if (rc != CUSAT_OK) throw std::runtime_error(std::string("rx_fec_chain: ") + what + ": " + cusat_last_error());
 }
+
+size_t round256(size_t n) { return (n + 255) & ~(size_t)255; }
 
 float snr_ratio(float signal, float error)
 {
@@ -113,34 +116,55 @@ rx_fec_chain::rx_fec_chain(fecframe_size_t framesize, code_rate_t rate, modulati
     d_output.resize((size_t)d_kbch_bytes * d_batch_size);
     d_flight.resize(N_FLIGHT);
     d_streams.resize(d_n_streams);
+    // frames start on 64-byte boundaries in the slots and on the device
+    d_sym_stride = (d_xfecframe_len + 7) & ~7;
+    const size_t mb = (size_t)d_micro_batch;
+    d_res_bytes = round256(3 * mb * sizeof(int32_t));
+    const size_t sym_bytes = mb * d_sym_stride * sizeof(complexf);
+    const size_t blk_bytes = d_res_bytes + mb * d_max_dfl;
     try {
-        const size_t mb = (size_t)d_micro_batch;
+        abi_check(cusat_fec_ctx_create(&d_ctx, (int)framesize, (int)rate, (int)constellation, max_trials, 1, 0),
+                  "context");
         for (auto& r : d_streams) {
-            hipStream_t s = nullptr;
-            hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "stream");
-            r.stream = s;
-            abi_check(cusat_fec_ctx_create(&r.ctx, (int)framesize, (int)rate, (int)constellation, max_trials,
-                                           d_micro_batch, 0),
-                      "context");
-            hip_check(hipMalloc(&r.sym_dev, mb * d_xfecframe_len * sizeof(complexf)), "device buffers");
-            hip_check(hipMalloc(&r.llr_dev, mb * d_nldpc * sizeof(int16_t)), "device buffers");
-            hip_check(hipMalloc(&r.hard_dev, mb * d_kldpc_bytes), "device buffers");
-            hip_check(hipMalloc(&r.res_dev, 3 * mb * sizeof(int32_t)), "device buffers");
-            hip_check(hipMalloc(&r.gse_dev, mb * d_max_dfl), "device buffers");
+            hipStream_t st = nullptr;
+            hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
+            r.stream = st;
+            abi_check(cusat_stream_create(&r.sc, &d_ctx, 1, d_micro_batch, 0), "stream context");
+            hip_check(hipMalloc(&r.sym_dev, sym_bytes), "device buffers");
+            hip_check(hipMalloc(&r.blk, blk_bytes), "device buffers");
+            // capture the three receive launches once; every micro-batch
+            // replays them with its own plan
+            abi_check(cusat_stream_rx_plan_async(r.sc, nullptr, 0, nullptr, 0, st), "plan");
+            int32_t* res = reinterpret_cast<int32_t*>(r.blk);
+            const float* sym = reinterpret_cast<const float*>(r.sym_dev);
+            auto rx = [&] {
+                return cusat_stream_rx(r.sc, sym, r.blk + d_res_bytes, res, res + mb, res + 2 * mb, st);
+            };
+            abi_check(rx(), "rx");
+            hip_check(hipStreamSynchronize(st), "warm launch");
+            hipGraph_t g = nullptr;
+            hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "capture");
+            const int rc = rx();
+            const hipError_t ec = hipStreamEndCapture(st, &g);
+            abi_check(rc, "capture");
+            hip_check(ec, "capture");
+            hipGraphExec_t ge = nullptr;
+            const hipError_t ei = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            hip_check(ei, "graph instantiate");
+            r.graph = ge;
         }
         for (auto& f : d_flight) {
-            hip_check(hipHostMalloc((void**)&f.syms, mb * d_xfecframe_len * sizeof(complexf), hipHostMallocDefault),
-                      "pinned slots");
-            hip_check(hipHostMalloc((void**)&f.trials, 3 * mb * sizeof(int32_t), hipHostMallocDefault),
-                      "pinned slots");
+            hip_check(hipHostMalloc((void**)&f.syms, sym_bytes, hipHostMallocDefault), "pinned slots");
+            hip_check(hipHostMalloc((void**)&f.blk, blk_bytes, hipHostMallocDefault), "pinned slots");
+            f.trials = reinterpret_cast<int32_t*>(f.blk);
             f.valid = f.trials + mb;
             f.dfl = f.trials + 2 * mb;
-            hip_check(hipHostMalloc((void**)&f.gse, mb * d_max_dfl, hipHostMallocDefault), "pinned slots");
+            f.gse = f.blk + d_res_bytes;
             // touch every page now: a first write into pinned memory costs a
             // fault per page, which would land in submit_xfecframe
-            memset(f.syms, 0, mb * d_xfecframe_len * sizeof(complexf));
-            memset(f.trials, 0, 3 * mb * sizeof(int32_t));
-            memset(f.gse, 0, mb * d_max_dfl);
+            memset(f.syms, 0, sym_bytes);
+            memset(f.blk, 0, blk_bytes);
             hipEvent_t ev = nullptr;
             hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "events");
             f.event = ev;
@@ -158,21 +182,20 @@ void rx_fec_chain::free_all()
         if (r.stream) hipStreamSynchronize(r.stream);
     for (auto& f : d_flight) {
         if (f.syms) hipHostFree(f.syms);
-        if (f.trials) hipHostFree(f.trials);
-        if (f.gse) hipHostFree(f.gse);
+        if (f.blk) hipHostFree(f.blk);
         if (f.event) hipEventDestroy(f.event);
         f = flight_slot{};
     }
     for (auto& r : d_streams) {
+        if (r.graph) hipGraphExecDestroy(r.graph);
+        if (r.sc) cusat_stream_destroy(r.sc);
         if (r.sym_dev) hipFree(r.sym_dev);
-        if (r.llr_dev) hipFree(r.llr_dev);
-        if (r.hard_dev) hipFree(r.hard_dev);
-        if (r.res_dev) hipFree(r.res_dev);
-        if (r.gse_dev) hipFree(r.gse_dev);
-        if (r.ctx) cusat_fec_ctx_destroy(r.ctx);
+        if (r.blk) hipFree(r.blk);
         if (r.stream) hipStreamDestroy(r.stream);
         r = stream_res{};
     }
+    if (d_ctx) cusat_fec_ctx_destroy(d_ctx);
+    d_ctx = nullptr;
 }
 
 rx_fec_chain::~rx_fec_chain()
@@ -219,8 +242,9 @@ void rx_fec_chain::reset()
 bool rx_fec_chain::submit_xfecframe(const complexf* symbols, int n_symbols)
 {
     bool ready = false;
-    // drain at micro-batch boundaries and when the ring fills up
-    if (d_batch_idx == 0 || d_submit_idx - d_drain_idx >= N_FLIGHT / 2) {
+    // drain at micro-batch boundaries, when the ring fills up and while a
+    // closed micro-batch waits for decode budget (a drain releases budget)
+    if (d_batch_idx == 0 || d_launch_idx < d_submit_idx || d_submit_idx - d_drain_idx >= N_FLIGHT / 2) {
         d_output_len = 0;
         d_gse_dfl_sizes.clear();
         ready = drain_completed();
@@ -236,8 +260,8 @@ bool rx_fec_chain::submit_xfecframe(const complexf* symbols, int n_symbols)
         return ready;
     }
     flight_slot& slot = d_flight[d_submit_idx % N_FLIGHT];
-    complexf* dst = slot.syms + (size_t)d_batch_idx * d_xfecframe_len;
-    memcpy(dst, symbols, (size_t)n_symbols * sizeof(complexf));
+    complexf* dst = slot.syms + (size_t)d_batch_idx * d_sym_stride;
+    detail::copy_frame(dst, symbols, (size_t)n_symbols * sizeof(complexf));
     if (++d_snr_skip_count >= 32) {
         d_snr_skip_count = 0;
         update_dd_snr(symbols);  // the caller's copy: pinned memory is slow to read from the host
@@ -349,23 +373,32 @@ void rx_fec_chain::launch_slot(flight_slot& slot)
     const int si = d_next_stream++ % d_n_streams;
     stream_res& r = d_streams[si];
     const int n = slot.n_frames;
-    const size_t mb = (size_t)d_micro_batch;
+    cusat_stream_rx_frame recs[64];
+    std::vector<cusat_stream_rx_frame> big;
+    cusat_stream_rx_frame* rec = recs;
+    if (n > 64) {
+        big.resize(n);
+        rec = big.data();
+    }
+    for (int i = 0; i < n; i++) {
+        rec[i].code = 0;
+        rec[i].flags = 0;
+        rec[i].sym_offset = (int64_t)i * d_sym_stride;
+        rec[i].out_offset = (int64_t)i * d_max_dfl;
+        rec[i].noise_var = d_noise_var;  // the chain's scaling when the micro-batch launches
+        rec[i].llr_gain = d_llr_gain;
+    }
     {
         std::unique_lock<std::mutex> lk;
         if (d_mtx) lk = std::unique_lock<std::mutex>(*d_mtx);
         hipStream_t st = r.stream;
-        hip_check(hipMemcpyAsync(r.sym_dev, slot.syms, (size_t)n * d_xfecframe_len * sizeof(complexf),
-                                 hipMemcpyHostToDevice, st),
-                  "copy in");
-        abi_check(cusat_rx_fec_batch(r.ctx, reinterpret_cast<const float*>(r.sym_dev), r.llr_dev, n, d_noise_var,
-                                     d_llr_gain, r.hard_dev, r.res_dev, r.res_dev + mb, st),
-                  "decode");
-        abi_check(cusat_bb_extract(r.ctx, r.hard_dev, r.res_dev, r.res_dev + mb, r.gse_dev, r.res_dev + 2 * mb, n,
-                                   st),
-                  "baseband");
-        hip_check(hipMemcpyAsync(slot.trials, r.res_dev, 3 * mb * sizeof(int32_t), hipMemcpyDeviceToHost, st),
+        const size_t sym_bytes = ((size_t)(n - 1) * d_sym_stride + d_xfecframe_len) * sizeof(complexf);
+        hip_check(hipMemcpyAsync(r.sym_dev, slot.syms, sym_bytes, hipMemcpyHostToDevice, st), "copy in");
+        abi_check(cusat_stream_rx_plan_async(r.sc, rec, n, nullptr, 0, st), "plan");
+        hip_check(hipGraphLaunch(r.graph, st), "graph launch");
+        // trials, valid, data-field lengths and data fields in one copy
+        hip_check(hipMemcpyAsync(slot.blk, r.blk, d_res_bytes + (size_t)n * d_max_dfl, hipMemcpyDeviceToHost, st),
                   "copy out");
-        hip_check(hipMemcpyAsync(slot.gse, r.gse_dev, (size_t)n * d_max_dfl, hipMemcpyDeviceToHost, st), "copy out");
         hip_check(hipEventRecord(slot.event, st), "event");
     }
     slot.in_flight = true;

@@ -9,14 +9,13 @@
 
 #include <hip/hip_runtime_api.h>
 
-#include <immintrin.h>
-
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
 #include <string>
 
+#include "copy_frame.h"
 #include "cusat_fec.h"
 
 namespace cusat_dsp {
@@ -33,38 +32,6 @@ void abi_check(int rc, const char* what)
 {
     if (rc != CUSAT_OK)
         throw std::runtime_error(std::string("rx_fec_stream_pool: ") + what + ": " + cusat_last_error());
-}
-
-// A frame's symbols into its pinned slot (32-byte aligned).  The slot is
-// written once and then read by the DMA engine, so streaming stores skip
-// the read-for-ownership of every destination line; frames are far below
-// the size at which memcpy switches to such stores itself.
-__attribute__((target("avx2"))) void copy_stream_avx2(void* dst, const void* src, size_t bytes)
-{
-    char* d = static_cast<char*>(dst);
-    const char* s = static_cast<const char*>(src);
-    size_t i = 0;
-    for (; i + 128 <= bytes; i += 128) {
-        const __m256i a = _mm256_loadu_si256(reinterpret_cast<const __m256i*>(s + i));
-        const __m256i b = _mm256_loadu_si256(reinterpret_cast<const __m256i*>(s + i + 32));
-        const __m256i c = _mm256_loadu_si256(reinterpret_cast<const __m256i*>(s + i + 64));
-        const __m256i e = _mm256_loadu_si256(reinterpret_cast<const __m256i*>(s + i + 96));
-        _mm256_stream_si256(reinterpret_cast<__m256i*>(d + i), a);
-        _mm256_stream_si256(reinterpret_cast<__m256i*>(d + i + 32), b);
-        _mm256_stream_si256(reinterpret_cast<__m256i*>(d + i + 64), c);
-        _mm256_stream_si256(reinterpret_cast<__m256i*>(d + i + 96), e);
-    }
-    _mm_sfence();  // the streaming stores are visible before the slot is handed to the GPU
-    if (i < bytes) memcpy(d + i, s + i, bytes - i);
-}
-
-void copy_frame(void* dst, const void* src, size_t bytes)
-{
-    static const bool avx2 = __builtin_cpu_supports("avx2");
-    if (avx2)
-        copy_stream_avx2(dst, src, bytes);
-    else
-        memcpy(dst, src, bytes);
 }
 
 size_t round256(size_t n) { return (n + 255) & ~(size_t)255; }
@@ -298,8 +265,10 @@ bool rx_fec_stream_pool::submit(const complexf* symbols, int n_symbols, const pl
     auto it = d_by_idx.find(pls.modcod);
     if (it == d_by_idx.end()) return false;
     modcod_state& m = d_mc[it->second];
-    // drain at micro-batch boundaries and when the ring fills up
-    if (d_batch_idx == 0 || d_submit_idx - d_drain_idx >= d_n_flight / 2) drain(false);
+    // drain at micro-batch boundaries, when the ring fills up and while a
+    // closed micro-batch waits for decode budget (a drain releases budget)
+    if (d_batch_idx == 0 || d_launch_idx < d_submit_idx || d_submit_idx - d_drain_idx >= d_n_flight / 2)
+        drain(false);
     while (d_block_when_full && d_submit_idx - d_drain_idx >= d_n_flight - 2) drain(true, true);
     if (d_submit_idx - d_drain_idx >= d_n_flight - 2 || n_symbols != m.xfec || !symbols) {
         d_gpu_drop_cnt++;
@@ -317,7 +286,7 @@ bool rx_fec_stream_pool::submit(const complexf* symbols, int n_symbols, const pl
     }
     s.n_syms = (s.n_syms + 7) & ~(int64_t)7;  // 64-byte aligned frame starts
     complexf* dst = s.syms + s.n_syms;
-    copy_frame(dst, symbols, (size_t)n_symbols * sizeof(complexf));
+    detail::copy_frame(dst, symbols, (size_t)n_symbols * sizeof(complexf));
     if (++m.snr_skip >= 32) {
         m.snr_skip = 0;
         if (!d_gpu_dd) {

@@ -5,10 +5,11 @@
  * same constructor, methods, statistics and static acceptance test).
  *
  * Underneath, every micro-batch is one call sequence of the C ABI
- * (include/cusat_fec.h) on one of the chain's HIP streams:
- * host->device copy of the symbols, cusat_rx_fec_batch (demap + LDPC + BCH),
- * cusat_bb_extract (BB descramble + header + GSE data field, on the GPU),
- * device->host copy of the per-frame results into the flight slot.
+ * (include/cusat_fec.h) on one of the chain's HIP streams: host->device copy
+ * of the symbols, cusat_stream_rx_plan_async, one replay of a captured
+ * cusat_stream_rx graph (demap, LDPC + BCH, BB descramble + header + GSE
+ * data field, all on the GPU), one device->host copy of the per-frame
+ * results into the flight slot.
  * Frames are submitted into a ring of pinned flight slots, launched when a
  * micro-batch is full (subject to the shared decode budget) and drained in
  * submission order (rx_fec_chain.cc:180-392).
@@ -32,8 +33,10 @@
 #include "modcod_types.h"
 
 struct cusat_fec_ctx;
+struct cusat_stream_ctx;
 struct ihipStream_t;
 struct ihipEvent_t;
+struct hipGraphExec;
 
 namespace cusat_dsp {
 
@@ -121,11 +124,13 @@ private:
     static constexpr int N_FLIGHT = 128;
 
     struct flight_slot {
-        complexf* syms = nullptr;   // pinned, micro_batch x xfecframe
-        int32_t* trials = nullptr;  // pinned, micro_batch
+        complexf* syms = nullptr;   // pinned, micro_batch x xfecframe (64-byte aligned frames)
+        // one pinned block, the layout of stream_res::blk
+        unsigned char* blk = nullptr;
+        int32_t* trials = nullptr;  // micro_batch each
         int32_t* valid = nullptr;
         int32_t* dfl = nullptr;     // data-field bytes (-1 = FEC failure)
-        uint8_t* gse = nullptr;     // pinned, micro_batch x max data field
+        uint8_t* gse = nullptr;     // micro_batch x max data field
         ihipEvent_t* event = nullptr;
         bool in_flight = false;
         int stream_idx = -1;
@@ -133,18 +138,19 @@ private:
     };
     struct stream_res {
         ihipStream_t* stream = nullptr;
-        cusat_fec_ctx* ctx = nullptr;
-        float* sym_dev = nullptr;
-        int16_t* llr_dev = nullptr;
-        uint8_t* hard_dev = nullptr;
-        int32_t* res_dev = nullptr;  // trials | valid | dfl, micro_batch each
-        uint8_t* gse_dev = nullptr;
+        cusat_stream_ctx* sc = nullptr;  // over the chain's one context
+        complexf* sym_dev = nullptr;
+        unsigned char* blk = nullptr;    // trials | valid | dfl | data fields
+        hipGraphExec* graph = nullptr;   // cusat_stream_rx
     };
 
     modulation_t d_modulation;
     int d_n_mod = 0;
     std::vector<complexf> d_points;  // constellation, for decision-directed SNR
     int d_kbch_bytes = 0, d_max_dfl = 0, d_kldpc_bytes = 0, d_nldpc = 0, d_xfecframe_len = 0;
+    int d_sym_stride = 0;     // symbols per frame slot (xfecframe rounded up to 8)
+    size_t d_res_bytes = 0;   // trials | valid | dfl in a result block
+    cusat_fec_ctx* d_ctx = nullptr;
     int d_batch_size = 0, d_batch_idx = 0, d_micro_batch = 4, d_n_streams = 8;
     std::vector<uint8_t> d_output;
     int d_output_len = 0;
