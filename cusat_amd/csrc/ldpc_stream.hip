@@ -96,28 +96,120 @@ int split_variant_id(int cmax, bool exact)
     return -1;
 }
 
-// 12-wave workgroups: the split-edge decoder; 6-wave: decode_frame
-template <bool WIDE>
-__device__ __forceinline__ int decode_variant(const code_desc& c, unsigned char* lds, const int16_t* llr,
-                                              uint4* rec, uint8_t* h, const int* beta, const int* alpha,
-                                              const ldpc_dev::kparams& p)
+// The LDS of the stream kernels, at namespace scope so that the decoder
+// functions below address it directly (LDS instructions, not flat ones):
+// the whole LDS as a static array (FIXED kernels: address known at compile
+// time, no base add on every access) or the dynamic allocation.
+__shared__ __align__(16) unsigned char stream_lds_fixed[ldpc_dev::SPLIT_LDS_STATIC];
+extern __shared__ __align__(16) unsigned char stream_lds_dyn[];
+
+template <bool FIXED>
+__device__ __forceinline__ unsigned char* stream_lds()
 {
+    if constexpr (FIXED)
+        return stream_lds_fixed;
+    else
+        return stream_lds_dyn;
+}
+
+// Address spaces for pointers that cross into the decoder functions.  A
+// kernel knows its pointer arguments (and pointers loaded through them) are
+// global; a device function sees generic pointers and would use flat
+// instructions, and no scalar loads.  The read-only tables are constant
+// memory (uniform loads become scalar loads), outputs global memory.
+#define CUSAT_AS_GLOBAL __attribute__((address_space(1)))
+#define CUSAT_AS_CONST __attribute__((address_space(4)))
+
+// code_desc as the decoder functions read it: the same layout, the tables
+// in constant memory
+struct code_desc_dev {
+    CUSAT_AS_CONST const ldpc_dev::circ_rec* circ;
+    CUSAT_AS_CONST const int* lay_cnt;
+    CUSAT_AS_CONST const int* lay_off;
+    CUSAT_AS_CONST const int* steps;
+    CUSAT_AS_CONST const int* grp_off;
+    CUSAT_AS_CONST const int* step_defer;
+    CUSAT_AS_CONST const int* groups;
+    CUSAT_AS_CONST const int* beta;
+    CUSAT_AS_CONST const int* alpha;
+    CUSAT_AS_CONST const uint64_t* bch_lut;
+    CUSAT_AS_CONST const uint64_t* bch_pw;
+    int N, K, R, q, n_steps, n_slots, n_groups;
+    int max_trials, learned_trials, n_param_iters;
+    int variant;
+    int variant_split;
+    int bch_bytes, bch_degree, bch_chunk, bch_words;
+};
+static_assert(sizeof(code_desc_dev) == sizeof(code_desc), "code_desc_dev mirrors code_desc");
+
+// Function arguments arrive in VGPRs; wave-uniform values are made uniform
+// again so that loads through them stay scalar and branches on them stay
+// scalar branches.
+template <class T>
+__device__ __forceinline__ T* uniform(T* ptr)
+{
+    const uint64_t v = reinterpret_cast<uint64_t>(ptr);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return reinterpret_cast<T*>(((uint64_t)hi << 32) | lo);
+}
+
+// One decoder variant of one frame, as a function of its own: every variant
+// gets its own register allocation, as in the per-variant batch kernels.
+// With all variants inlined into the stream kernel the allocator spilled
+// (over 100 VGPRs at the 12-wave register budget) and every code paid for
+// it.  SPLIT: 12-wave split-edge decoder, else the 6-wave decode_frame.
+template <bool SPLIT, int C, bool E, bool FIXED>
+__device__ __attribute__((noinline)) int decode_one(CUSAT_AS_CONST const code_desc_dev* codes_v, int ci_v,
+                                                    CUSAT_AS_CONST const int16_t* llr_v, CUSAT_AS_GLOBAL uint4* rec_v,
+                                                    CUSAT_AS_GLOBAL uint8_t* h_v, int lp_v)
+{
+    const auto& c = uniform(codes_v)[__builtin_amdgcn_readfirstlane(ci_v)];
+    const bool lp = __builtin_amdgcn_readfirstlane(lp_v) != 0;
+    const ldpc_dev::kparams p{c.N,      c.K, c.R, c.q, c.n_steps, lp ? c.learned_trials : c.max_trials,
+                              lp ? c.n_param_iters : 0, c.n_slots, c.n_groups};
+    // generic pointers for the decoder; once it is inlined here, the
+    // compiler carries the address spaces through to every access
+    const int16_t* llr = (const int16_t*)uniform(llr_v);
+    uint4* rec = (uint4*)uniform(rec_v);
+    uint8_t* h = (uint8_t*)uniform(h_v);
+    const auto* circ = (const ldpc_dev::circ_rec*)c.circ;
+    const int* beta = lp ? (const int*)c.beta : nullptr;
+    const int* alpha = lp ? (const int*)c.alpha : nullptr;
+    if constexpr (SPLIT)
+        return ldpc_dev::decode_frame_split<C, E, 2>(stream_lds<FIXED>(), llr, rec, h, circ, (const int*)c.lay_cnt,
+                                                     (const int*)c.lay_off, (const int*)c.steps,
+                                                     (const int*)c.grp_off, (const int*)c.step_defer,
+                                                     (const int*)c.groups, beta, alpha, p);
+    else
+        return ldpc_dev::decode_frame<C, E, 2>(stream_lds<FIXED>(), llr, rec, h, circ, (const int*)c.lay_cnt,
+                                               (const int*)c.lay_off, (const int*)c.steps, (const int*)c.grp_off,
+                                               (const int*)c.groups, beta, alpha, p);
+}
+
+// 12-wave workgroups: the split-edge decoder; 6-wave: decode_frame
+template <bool WIDE, bool FIXED>
+__device__ __forceinline__ int decode_variant(const code_desc* codes_g, int ci, const code_desc& c,
+                                              const int16_t* llr_g, uint4* rec_g, uint8_t* h_g, int lp)
+{
+    const auto* codes = (CUSAT_AS_CONST const code_desc_dev*)codes_g;
+    const auto* llr = (CUSAT_AS_CONST const int16_t*)llr_g;
+    auto* rec = (CUSAT_AS_GLOBAL uint4*)rec_g;
+    auto* h = (CUSAT_AS_GLOBAL uint8_t*)h_g;
     if (WIDE) {
         switch (c.variant_split) {
-#define X(ID, C, E)                                                                                         \
-    case ID:                                                                                                \
-        return ldpc_dev::decode_frame_split<C, E, 2>(lds, llr, rec, h, c.circ, c.lay_cnt, c.lay_off, c.steps, \
-                                                     c.grp_off, c.step_defer, c.groups, beta, alpha, p);
+#define X(ID, C, E) \
+    case ID:        \
+        return decode_one<true, C, E, FIXED>(codes, ci, llr, rec, h, lp);
             CUSAT_STREAM_SPLIT_VARIANTS(X)
 #undef X
         }
         return -1;
     }
     switch (c.variant) {
-#define X(ID, C, E)                                                                                      \
-    case ID:                                                                                             \
-        return ldpc_dev::decode_frame<C, E, 2>(lds, llr, rec, h, c.circ, c.lay_cnt, c.lay_off, c.steps, \
-                                               c.grp_off, c.groups, beta, alpha, p);
+#define X(ID, C, E) \
+    case ID:        \
+        return decode_one<false, C, E, FIXED>(codes, ci, llr, rec, h, lp);
         CUSAT_STREAM_VARIANTS(X)
 #undef X
     }
@@ -208,14 +300,7 @@ __global__ void __launch_bounds__(stream_threads<WIDE>())
                        int32_t* __restrict__ fallback, uint4* __restrict__ ws, size_t ws_v4, int learned,
                        int ctl_off)
 {
-    unsigned char* lds_raw;
-    if constexpr (FIXED) {
-        __shared__ __align__(16) unsigned char lds_fixed[ldpc_dev::SPLIT_LDS_STATIC];
-        lds_raw = lds_fixed;
-    } else {
-        extern __shared__ __align__(16) unsigned char lds_dyn[];
-        lds_raw = lds_dyn;
-    }
+    unsigned char* lds_raw = stream_lds<FIXED>();
     int& job_s = reinterpret_cast<int*>(lds_raw + ctl_off)[0];
     int& flag_s = reinterpret_cast<int*>(lds_raw + ctl_off)[1];
     uint4* rec = ws + (size_t)blockIdx.x * ws_v4;
@@ -236,11 +321,7 @@ __global__ void __launch_bounds__(stream_threads<WIDE>())
             int res = -1, ok = 0, fb = 0;
             for (int pass = 0;; pass++) {
                 const bool lp = use_learned && pass == 0;
-                const ldpc_dev::kparams p{c.N, c.K, c.R, c.q, c.n_steps,
-                                          lp ? c.learned_trials : c.max_trials,
-                                          lp ? c.n_param_iters : 0, c.n_slots, c.n_groups};
-                res = decode_variant<WIDE>(c, lds_raw, fl, rec, fh, lp ? c.beta : nullptr,
-                                           lp ? c.alpha : nullptr, p);
+                res = decode_variant<WIDE, FIXED>(codes, ci, c, fl, rec, fh, lp ? 1 : 0);
                 ok = bch_frame<stream_threads<WIDE>()>(lds_raw, &flag_s, fh, c);
                 if (!lp || (res >= 0 && ok)) break;
                 fb = 1;
