@@ -4,8 +4,8 @@
 // One wave64 per frame.  Lane l folds its byte chunk into
 // r_l = chunk_l(x) x^P mod g with the byte table lut[b] = b(x) x^P mod g
 // (kept in LDS), then moves it to its place in the frame with
-// r_l x^(8 (n - end_l)) mod g = XOR of pw[l][bit] over the set bits of r_l
-// (pw[l][b] = x^(8 (n - end_l) + b) mod g, precomputed per geometry), and the
+// r_l x^(8 (n - end_l)) mod g = XOR of pw[bit][l] over the set bits of r_l
+// (pw[b][l] = x^(8 (n - end_l) + b) mod g, precomputed per geometry), and the
 // wave XOR-reduces the 64 partial remainders with cross-lane shuffles.
 // Because g(0) = 1, c(x) x^P mod g == 0 exactly when c(x) mod g == 0.
 #include <hip/hip_runtime.h>
@@ -48,13 +48,19 @@ __global__ void __launch_bounds__(256) bch_check_kernel(bch_args a, int batch)
     uint64_t acc[W];
 #pragma unroll
     for (int w = 0; w < W; w++) acc[w] = 0;
-    if (begin < end) {
-        const uint64_t* pw = a.pw + (size_t)lane * P * W;
-        for (int b = 0; b < P; b++) {
-            if ((r[b >> 6] >> (b & 63)) & 1) {
+    // pw rows are [bit][word][lane]: one row's load is 512 contiguous bytes
+    // for the wave.  Branch-free (a lane without bytes has r = 0, and its
+    // table entries are 0); the waves of the launch hide the load latency.
+    const uint64_t* pw = a.pw + lane;
 #pragma unroll
-                for (int w = 0; w < W; w++) acc[w] ^= pw[b * W + w];
-            }
+    for (int wi = 0; wi < W; wi++) {
+        const int nb = min(64, P - 64 * wi);
+#pragma unroll 8
+        for (int bb = 0; bb < nb; bb++) {
+            const uint64_t m = 0ull - ((r[wi] >> bb) & 1ull);
+            const size_t row = (size_t)(64 * wi + bb) * W;
+#pragma unroll
+            for (int w = 0; w < W; w++) acc[w] ^= pw[(row + w) * 64] & m;
         }
     }
 #pragma unroll

@@ -65,7 +65,7 @@ int upload_tables(cusat_fec_ctx* c)
         const size_t end = std::min(n_bytes, begin + c->bch_chunk);
         const size_t shift = (n_bytes - end) * 8;
         for (int b = 0; b < c->bch.degree; b++)
-            for (int w = 0; w < W; w++) pw[((size_t)l * c->bch.degree + b) * W + w] = powers[shift + b][w];
+            for (int w = 0; w < W; w++) pw[((size_t)b * W + w) * 64 + l] = powers[shift + b][w];  // [bit][word][lane]
     }
 
     // schedule as int32 (scalar loads), padded to an even step count with an

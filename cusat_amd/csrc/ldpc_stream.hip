@@ -243,13 +243,27 @@ __device__ __forceinline__ int bch_wave(const uint8_t* __restrict__ src, const u
     uint64_t acc[W];
 #pragma unroll
     for (int w = 0; w < W; w++) acc[w] = 0;
-    if (begin < end) {
-        const uint64_t* pl = pw + (size_t)lane * P * W;
-        for (int b = 0; b < P; b++)
-            if ((r[b >> 6] >> (b & 63)) & 1) {
+    // pw rows are [bit][word][lane] (one row's load is 512 contiguous bytes
+    // for the wave).  Branch-free, 8 rows' loads issued before their uses.
+    const uint64_t* pl = pw + lane;
 #pragma unroll
-                for (int w = 0; w < W; w++) acc[w] ^= pl[b * W + w];
+    for (int wi = 0; wi < W; wi++) {
+        const int nb = min(64, P - 64 * wi);
+        for (int b0 = 0; b0 < nb; b0 += 8) {
+            uint64_t v[8][W];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const size_t row = (size_t)(64 * wi + b0 + j) * W;
+#pragma unroll
+                for (int w = 0; w < W; w++) v[j][w] = b0 + j < nb ? pl[(row + w) * 64] : 0ull;
             }
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const uint64_t m = 0ull - ((r[wi] >> (b0 + j)) & 1ull);
+#pragma unroll
+                for (int w = 0; w < W; w++) acc[w] ^= v[j][w] & m;
+            }
+        }
     }
     uint64_t any = 0;
 #pragma unroll
@@ -265,19 +279,20 @@ __device__ __forceinline__ int bch_wave(const uint8_t* __restrict__ src, const u
 // workgroup before the call); returns 1 = valid, the same value in every thread.
 template <int NT>
 __device__ __forceinline__ int bch_frame(unsigned char* lds, int* flag, const uint8_t* __restrict__ h,
-                                         const code_desc& c)
+                                         const uint64_t* __restrict__ lut, const uint64_t* __restrict__ pw,
+                                         int n_bytes, int degree, int chunk, int words)
 {
     uint64_t* tab = reinterpret_cast<uint64_t*>(lds);
     __syncthreads();  // the decoder is done with the LDS; h is written
-    for (int k = threadIdx.x; k < 256 * c.bch_words; k += NT) tab[k] = c.bch_lut[k];
+    for (int k = threadIdx.x; k < 256 * words; k += NT) tab[k] = lut[k];
     __syncthreads();
     if (threadIdx.x < 64) {
         int ok = 0;
-        switch (c.bch_words) {
-        case 1: ok = bch_wave<1>(h, tab, c.bch_pw, c.bch_bytes, c.bch_degree, c.bch_chunk, threadIdx.x); break;
-        case 2: ok = bch_wave<2>(h, tab, c.bch_pw, c.bch_bytes, c.bch_degree, c.bch_chunk, threadIdx.x); break;
-        case 3: ok = bch_wave<3>(h, tab, c.bch_pw, c.bch_bytes, c.bch_degree, c.bch_chunk, threadIdx.x); break;
-        default: ok = bch_wave<4>(h, tab, c.bch_pw, c.bch_bytes, c.bch_degree, c.bch_chunk, threadIdx.x); break;
+        switch (words) {
+        case 1: ok = bch_wave<1>(h, tab, pw, n_bytes, degree, chunk, threadIdx.x); break;
+        case 2: ok = bch_wave<2>(h, tab, pw, n_bytes, degree, chunk, threadIdx.x); break;
+        case 3: ok = bch_wave<3>(h, tab, pw, n_bytes, degree, chunk, threadIdx.x); break;
+        default: ok = bch_wave<4>(h, tab, pw, n_bytes, degree, chunk, threadIdx.x); break;
         }
         if (threadIdx.x == 0) *flag = ok;
     }
@@ -285,6 +300,21 @@ __device__ __forceinline__ int bch_frame(unsigned char* lds, int* flag, const ui
     const int ok = *flag;
     __syncthreads();  // flag read by all before it is reused
     return ok;
+}
+
+// The BCH check of one frame as a function of its own, for the same
+// reasons as decode_one: inside the kernel body, at the kernel's register
+// limit, the table loads were generic (flat) and each waited for the one
+// before.  ctl_off: the byte offset of the control ints in the LDS.
+template <int NT, bool FIXED>
+__device__ __attribute__((noinline)) int bch_one(CUSAT_AS_CONST const code_desc_dev* codes_v, int ci_v,
+                                                 CUSAT_AS_GLOBAL const uint8_t* h_v, int ctl_off_v)
+{
+    const auto& c = uniform(codes_v)[__builtin_amdgcn_readfirstlane(ci_v)];
+    unsigned char* lds = stream_lds<FIXED>();
+    int* flag = reinterpret_cast<int*>(lds + __builtin_amdgcn_readfirstlane(ctl_off_v)) + 1;
+    return bch_frame<NT>(lds, flag, (const uint8_t*)uniform(h_v), (const uint64_t*)c.bch_lut,
+                         (const uint64_t*)c.bch_pw, c.bch_bytes, c.bch_degree, c.bch_chunk, c.bch_words);
 }
 
 // ctl_off: byte offset in the LDS of the two control ints (queue slot, BCH
@@ -322,7 +352,8 @@ __global__ void __launch_bounds__(stream_threads<WIDE>())
             for (int pass = 0;; pass++) {
                 const bool lp = use_learned && pass == 0;
                 res = decode_variant<WIDE, FIXED>(codes, ci, c, fl, rec, fh, lp ? 1 : 0);
-                ok = bch_frame<stream_threads<WIDE>()>(lds_raw, &flag_s, fh, c);
+                ok = bch_one<stream_threads<WIDE>(), FIXED>((CUSAT_AS_CONST const code_desc_dev*)codes, ci,
+                                                            (CUSAT_AS_GLOBAL const uint8_t*)fh, ctl_off);
                 if (!lp || (res >= 0 && ok)) break;
                 fb = 1;
             }
