@@ -62,8 +62,8 @@ def parse():
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py measures one GPU: the path shards by frame index with no exchange "
-              "(see DESIGN.md); multi-GPU runs are not part of this build", file=sys.stderr)
+        print("bench.py measures one GPU: the path shards by frame index with no exchange; "
+              "multi-GPU runs are not part of this build", file=sys.stderr)
         sys.exit(2)
     import torch
     import cusat_amd as ca
