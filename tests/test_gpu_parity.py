@@ -149,3 +149,28 @@ def test_bch_single_bit_errors():
     v = gpu_bch(ctx, frames)
     np.testing.assert_array_equal(v, [0, 0, 0, 0, 1, 1, 1, 1])
     np.testing.assert_array_equal(v, oc.bch_check(frames, n, 0, 12))
+
+
+def _all_codes():
+    """(framesize, rate) of every LDPC code of the tables, with QPSK at
+    2.5 dB above the Shannon limit of the rate: the middle rates converge,
+    the lowest and highest mostly do not."""
+    import math
+    out = []
+    for fs in (ca.FECFRAME_NORMAL, ca.FECFRAME_SHORT):
+        for r in ca.RATES:
+            try:
+                ca.geometry(fs, r, "QPSK")
+            except Exception:
+                continue
+            a, b = r.split("/")
+            eta = 2.0 * int(a) / int(b)
+            out.append((fs, r, round(10.0 * math.log10(2.0 ** eta - 1.0) + 2.5, 2)))
+    return out
+
+
+@pytest.mark.parametrize("fs,rate,es", _all_codes())
+def test_every_code_against_oracle(fs, rate, es):
+    """All 51 LDPC codes (34 normal, 17 short) of the tables: demap, decode
+    and BCH bit-exact against the oracle on two frames each."""
+    _chain_parity(fs, rate, "QPSK", es, 2, 31)
