@@ -332,7 +332,6 @@ __global__ void __launch_bounds__(stream_threads<WIDE>())
 {
     unsigned char* lds_raw = stream_lds<FIXED>();
     int& job_s = reinterpret_cast<int*>(lds_raw + ctl_off)[0];
-    int& flag_s = reinterpret_cast<int*>(lds_raw + ctl_off)[1];
     uint4* rec = ws + (size_t)blockIdx.x * ws_v4;
     for (;;) {
         if (threadIdx.x == 0) job_s = (int)atomicAdd(&counters[0], 1u);
