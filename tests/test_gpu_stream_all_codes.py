@@ -40,12 +40,6 @@ def all_codes():
     return out
 
 
-def test_table_has_every_code():
-    codes = all_codes()
-    assert len(codes) == 51
-    assert sum(1 for c in codes if c[0] == ca.FECFRAME_SHORT) == 17
-
-
 @pytest.mark.parametrize("tuning", [None, {"wg_per_cu": 1, "frames_per_job": 1, "order": 1, "split_edges": 0}])
 def test_every_code_in_one_stream(tuning):
     mix = all_codes()

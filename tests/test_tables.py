@@ -175,3 +175,18 @@ def test_geometry_collision_plan(fs, rate, name):
     g_seq = ca.geometry(fs, rate, "QPSK", flags=ca.CTX_SEQUENTIAL_SCHEDULE)
     assert g_seq["lanes"] == 1 and g_seq["n_steps"] == g["q"]
     assert g_seq["n_collision_slots"] == census_plan(name, use_sched=False)
+
+
+def test_every_ldpc_code_enumerates():
+    """The GPU tests that sweep every code see all 51 LDPC codes of the
+    tables (34 normal, 17 short), each with its oracle table of equal size."""
+    from oracle import oracle_ctypes as oc
+    from tests.fec_util import TABLE
+    from tests.test_gpu_stream_all_codes import all_codes
+    codes = all_codes()
+    assert len(codes) == 51
+    assert sum(1 for c in codes if c[0] == ca.FECFRAME_SHORT) == 17
+    for fs, r, _, _ in codes:
+        g = ca.geometry(fs, r, "QPSK")
+        c = oc.Code(TABLE[(fs, r)])
+        assert (c.N, c.K) == (g["n_ldpc"], g["k_ldpc"]), (fs, r)
