@@ -18,7 +18,8 @@ sample of the same frames).
 
 The same line carries "learned_vs_plain", measured in the same run: the
 learned offset / normalised min-sum decoder with plain fallback against
-the plain decoder at their FER-1e-3 operating point.  The workload is
+the plain decoder at their FER-1e-3 operating point.  "learned_vs_plain_stream"
+does the same on the mixed DVB-S2/S2X stream (per-MODCOD parameters).  The workload is
 DVB-S2 normal 16APSK 2/3, 16384 fresh frames from the GPU generator at
 Es/N0 9.2 dB, and each decoder is one persistent launch over all of them
 (decode + BCH, the learned decoder re-decoding failing frames with the
@@ -186,6 +187,7 @@ def main():
     }
     if not args.no_learned:
         out["learned_vs_plain"] = learned_vs_plain(args.learned_frames, args.learned_esn0)
+        out["learned_vs_plain_stream"] = learned_vs_plain_stream()
     print(json.dumps(out))
 
 
@@ -205,6 +207,26 @@ def learned_vs_plain(frames, es):
                                         r["learned+fallback"]["identical_to_batched"])}
     res["learned_fallback"]["fallback_rate"] = round(r["learned+fallback"]["fallback_rate"], 6)
     res["speedup"] = round(res["learned_fallback"]["gbps_info"] / res["plain"]["gbps_info"], 4)
+    return res
+
+
+def learned_vs_plain_stream():
+    """The mixed DVB-S2/S2X stream (4096 frames over 15 MODCODs, normal and
+    short, threshold + 0.3 dB): per-MODCOD learned parameters with plain
+    fallback against the all-plain decoder, each one graph replay of the
+    persistent kernel over the same LLRs."""
+    from cusat_amd import stream_bench
+    quiet = lambda s: None  # noqa: E731
+    rl = stream_bench.run("s2x", reps=3, learned=True, log=quiet)
+    rp = stream_bench.run("s2x", reps=3, learned=False, log=quiet)
+    keep = ("gbps_info_stream", "fer", "mean_iterations", "ms_per_launch_graph")
+    res = {"workload": "mixed DVB-S2/S2X stream, 4096 frames over 15 MODCODs (25 codes), threshold + 0.3 dB",
+           "learned_codes": rl["learned_codes"],
+           "plain": {k: rp[k] for k in keep},
+           "learned_fallback": {k: rl[k] for k in keep},
+           "identical_to_per_modcod": bool(rl["identical_to_per_modcod"] and rp["identical_to_per_modcod"])}
+    res["learned_fallback"]["fallback_rate"] = rl["fallback_rate"]
+    res["speedup"] = round(rl["gbps_info_stream"] / rp["gbps_info_stream"], 4)
     return res
 
 
