@@ -99,12 +99,32 @@ def lib():
         L.cusat_stream_rx_plan_async.argtypes = [vp, vp, i, vp, i, vp]
         L.cusat_stream_rx_dd.argtypes = [vp, vp, vp, vp, vp]
         L.cusat_stream_rx.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.cusat_stream_rx_demap.argtypes = [vp, vp, vp]
+        L.cusat_stream_rx_decode.argtypes = [vp, vp, vp, vp, vp]
+        L.cusat_stream_rx_bb.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.cusat_stream_rx_work.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp)]
         L.cusat_stream_get_tuning.argtypes = [vp, ctypes.POINTER(StreamTuning), ctypes.POINTER(i)]
         L.cusat_stream_decode.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.cusat_stream_autotune.argtypes = [vp, vp, vp, vp, vp, vp, i, vp, ctypes.POINTER(StreamTuning),
                                             ctypes.POINTER(f)]
         _lib = L
     return _lib
+
+
+def copy_from_device(addr, nbytes):
+    """Host copy (uint8 array) of nbytes of device memory at the integer
+    address addr, e.g. a buffer of StreamContext.rx_work(); synchronous
+    hipMemcpy through the HIP runtime this library is linked to."""
+    L = lib()
+    fn = L.hipMemcpy
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    out = np.empty(int(nbytes), np.uint8)
+    if nbytes:
+        rc = fn(out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(int(addr)), int(nbytes), 4)  # hipMemcpyDefault
+        if rc != 0:
+            raise RuntimeError("hipMemcpy from device failed (%d)" % rc)
+    return out
 
 
 def _check(rc, what):
@@ -290,6 +310,26 @@ class StreamContext:
         """Demap, decode, BCH-check and BB-extract the current receive batch."""
         _check(lib().cusat_stream_rx(self._h, _ptr(sym), _ptr(out), _ptr(trials), _ptr(valid), _ptr(dfl),
                                      _stream(stream)), "cusat_stream_rx")
+
+    def rx_demap(self, sym, stream=None):
+        """First launch of rx(): the mixed demap into the LLR work buffer."""
+        _check(lib().cusat_stream_rx_demap(self._h, _ptr(sym), _stream(stream)), "cusat_stream_rx_demap")
+
+    def rx_decode(self, trials, valid, fallback=None, stream=None):
+        """Second launch of rx(): persistent decode + BCH of the LLR work buffer."""
+        _check(lib().cusat_stream_rx_decode(self._h, _ptr(trials), _ptr(valid), _ptr(fallback), _stream(stream)),
+               "cusat_stream_rx_decode")
+
+    def rx_bb(self, out, trials, valid, dfl, stream=None):
+        """Third launch of rx(): BB descramble + data-field extraction."""
+        _check(lib().cusat_stream_rx_bb(self._h, _ptr(out), _ptr(trials), _ptr(valid), _ptr(dfl),
+                                        _stream(stream)), "cusat_stream_rx_bb")
+
+    def rx_work(self):
+        """Device addresses (ints) of the rx LLR and hard-decision work buffers."""
+        a, b = ctypes.c_void_p(), ctypes.c_void_p()
+        _check(lib().cusat_stream_rx_work(self._h, ctypes.byref(a), ctypes.byref(b)), "cusat_stream_rx_work")
+        return a.value, b.value
 
     def tuning(self):
         t, m = StreamTuning(), ctypes.c_int()

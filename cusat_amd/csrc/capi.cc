@@ -31,6 +31,37 @@ void set_last_error(const char* fmt, ...)
     va_end(ap);
 }
 
+ldpc_args make_ldpc_args(const cusat_fec_ctx* c, const int16_t* llr, uint8_t* hard, int32_t* trials)
+{
+    ldpc_args a{};
+    a.llr = llr;
+    a.msg = c->d_msg;
+    a.hard = hard;
+    a.result = trials;
+    a.circ = c->d_circ;
+    a.lay_cnt = c->d_lay_cnt;
+    a.lay_off = c->d_lay_off;
+    a.lay_moff = c->d_lay_moff;
+    a.steps = c->d_steps;
+    a.grp_off = c->d_grp_off;
+    a.step_defer = c->d_step_defer;
+    a.groups = c->d_groups;
+    a.N = c->geo.N;
+    a.K = c->geo.K;
+    a.R = c->geo.R;
+    a.q = c->geo.q;
+    a.n_steps = c->kern_steps;
+    a.lanes = c->geo.lanes;
+    a.n_slots = c->geo.n_slots;
+    a.n_groups = (int)c->geo.groups.size();
+    a.lt_alloc = c->geo.lt_alloc;
+    a.max_trials = c->max_trials;
+    a.cmax = c->cmax;
+    a.exact = c->exact;
+    a.split = !(c->flags & CUSAT_CTX_NO_SPLIT_KERNEL);
+    return a;
+}
+
 }  // namespace cusat
 
 using namespace cusat;
@@ -126,37 +157,6 @@ int upload_tables(cusat_fec_ctx* c)
     HIP_TRY(hipMemcpy(c->d_tables, host.data(), total, hipMemcpyHostToDevice));
     for (size_t i = 0; i < pieces.size(); i++) *pieces[i].dst = (unsigned char*)c->d_tables + offs[i];
     return CUSAT_OK;
-}
-
-ldpc_args make_ldpc_args(const cusat_fec_ctx* c, const int16_t* llr, uint8_t* hard, int32_t* trials)
-{
-    ldpc_args a{};
-    a.llr = llr;
-    a.msg = c->d_msg;
-    a.hard = hard;
-    a.result = trials;
-    a.circ = c->d_circ;
-    a.lay_cnt = c->d_lay_cnt;
-    a.lay_off = c->d_lay_off;
-    a.lay_moff = c->d_lay_moff;
-    a.steps = c->d_steps;
-    a.grp_off = c->d_grp_off;
-    a.step_defer = c->d_step_defer;
-    a.groups = c->d_groups;
-    a.N = c->geo.N;
-    a.K = c->geo.K;
-    a.R = c->geo.R;
-    a.q = c->geo.q;
-    a.n_steps = c->kern_steps;
-    a.lanes = c->geo.lanes;
-    a.n_slots = c->geo.n_slots;
-    a.n_groups = (int)c->geo.groups.size();
-    a.lt_alloc = c->geo.lt_alloc;
-    a.max_trials = c->max_trials;
-    a.cmax = c->cmax;
-    a.exact = c->exact;
-    a.split = !(c->flags & CUSAT_CTX_NO_SPLIT_KERNEL);
-    return a;
 }
 
 bch_args make_bch_args(const cusat_fec_ctx* c, const uint8_t* hard, int32_t* valid)
@@ -334,6 +334,7 @@ extern "C" int cusat_fec_ctx_destroy(cusat_fec_ctx* c)
     if (c->d_msg) hipFree(c->d_msg);
     if (c->d_mask) hipFree(c->d_mask);
     if (c->d_beta) hipFree(c->d_beta);
+    for (int* p : c->retired_params) hipFree(p);
     if (c->d_gen) hipFree(c->d_gen);
     delete c;
     return CUSAT_OK;
@@ -428,9 +429,11 @@ extern "C" int cusat_ldpc_set_params(cusat_fec_ctx* c, const int16_t* beta, cons
             return CUSAT_E_INVALID;
         }
     if (n_iters > c->param_cap) {
+        // the outgrown buffer stays allocated until the context is
+        // destroyed: work enqueued before this call may still read it
         int* d = nullptr;
         HIP_TRY(hipMalloc(&d, 2 * n * sizeof(int)));
-        if (c->d_beta) hipFree(c->d_beta);
+        if (c->d_beta) c->retired_params.push_back(c->d_beta);
         c->d_beta = d;
         c->d_alpha = d + n;
         c->param_cap = n_iters;
@@ -448,6 +451,7 @@ extern "C" int cusat_ldpc_set_params(cusat_fec_ctx* c, const int16_t* beta, cons
         HIP_TRY(hipMemcpy(c->d_beta, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
     }
     c->n_param_iters = n_iters;
+    c->param_gen++;
     return CUSAT_OK;
 }
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "bch.h"
 #include "codes.h"
@@ -44,6 +45,13 @@ struct cusat_fec_ctx {
     int* d_beta = nullptr;
     int* d_alpha = nullptr;
     int n_param_iters = 0, param_cap = 0;
+    // bumped by every cusat_ldpc_set_params: a stream context re-uploads
+    // its copy of the parameter pointers and counts when it differs
+    unsigned param_gen = 0;
+    // parameter buffers outgrown by a larger table; a decode enqueued (or a
+    // graph captured) before the change may still read them, so they are
+    // freed with the context, not when replaced
+    std::vector<int*> retired_params;
     // test-traffic generator tables (cusat_gen_frames_dev), made on first use
     void* d_gen = nullptr;
     uint64_t* d_gen_pw = nullptr;

@@ -405,15 +405,21 @@ struct cusat_stream_ctx {
     // counters ([2] = job count), frames, jobs, order
     size_t off_jobs = 0, off_order = 0, plan_bytes = 0;
     static constexpr int N_STAGE = 4;
-    unsigned char* stage[N_STAGE] = {};  // pinned images for cusat_stream_plan_async
+    // pinned images for the async plan calls: the plan region, then the
+    // code descriptors (stage_bytes = plan_bytes + descriptors)
+    unsigned char* stage[N_STAGE] = {};
     hipEvent_t stage_done[N_STAGE] = {};
     bool stage_used[N_STAGE] = {};
     int next_stage = 0;
-    bool codes_uploaded = false;
+    size_t stage_bytes = 0;
+    // param_gen of each context when its descriptor was last uploaded: a
+    // plan call re-uploads the descriptors when a context's learned
+    // parameters changed since
+    std::vector<unsigned> code_gen;
     // receive batches (cusat_stream_rx): per-code records after d_codes, the
     // rx header (256 bytes: frames, demap blocks) and records at off_rx in the
-    // plan region, LLR and hard-decision work buffers (allocated by the first
-    // rx plan)
+    // plan region, LLR and hard-decision work buffers (all allocated by
+    // cusat_stream_create)
     rx_code* d_rx_codes = nullptr;
     size_t off_rx = 0, off_ev = 0;  // rx header + records; SNR events
     int max_events = 0;
@@ -493,9 +499,10 @@ int build_plan(cusat_stream_ctx* sc, const cusat_stream_tuning& tu)
     return CUSAT_OK;
 }
 
-int upload_codes(cusat_stream_ctx* sc)
+// The code descriptors as the kernel reads them, from the contexts' current
+// tables and learned parameters.
+void fill_code_descs(const cusat_stream_ctx* sc, code_desc* d)
 {
-    std::vector<code_desc> d(sc->codes.size());
     for (size_t k = 0; k < sc->codes.size(); k++) {
         const cusat_fec_ctx* c = sc->codes[k];
         code_desc& o = d[k];
@@ -528,7 +535,29 @@ int upload_codes(cusat_stream_ctx* sc)
         o.bch_chunk = c->bch_chunk;
         o.bch_words = (c->bch.degree + 63) / 64;
     }
+}
+
+bool params_changed(const cusat_stream_ctx* sc)
+{
+    for (size_t k = 0; k < sc->codes.size(); k++)
+        if (sc->codes[k]->param_gen != sc->code_gen[k]) return true;
+    return false;
+}
+
+void note_params(cusat_stream_ctx* sc)
+{
+    for (size_t k = 0; k < sc->codes.size(); k++) sc->code_gen[k] = sc->codes[k]->param_gen;
+}
+
+// Synchronous upload of the code descriptors (and, at creation, of the
+// receive-side per-code records).
+int upload_codes(cusat_stream_ctx* sc, bool rx_codes)
+{
+    std::vector<code_desc> d(sc->codes.size());
+    fill_code_descs(sc, d.data());
     HIP_TRY(hipMemcpy(sc->d_codes, d.data(), d.size() * sizeof(code_desc), hipMemcpyHostToDevice));
+    note_params(sc);
+    if (!rx_codes) return CUSAT_OK;
     std::vector<rx_code> r(sc->codes.size());
     for (size_t k = 0; k < sc->codes.size(); k++) {
         const cusat_fec_ctx* c = sc->codes[k];
@@ -550,6 +579,34 @@ int upload_codes(cusat_stream_ctx* sc)
         for (int b = 0; b < 8; b++) o.col[b] = c->il.col[b];
     }
     HIP_TRY(hipMemcpy(sc->d_rx_codes, r.data(), r.size() * sizeof(rx_code), hipMemcpyHostToDevice));
+    return CUSAT_OK;
+}
+
+// Take the next pinned staging image (waiting only if its previous copy has
+// not run yet) and, when a context's learned parameters changed since the
+// last upload, stage the new descriptors behind the plan and copy them on
+// `st`, ahead of the plan that will use them.
+int next_stage_image(cusat_stream_ctx* sc, hipStream_t st, unsigned char** img)
+{
+    const int k = sc->next_stage;
+    if (sc->stage_used[k]) HIP_TRY(hipEventSynchronize(sc->stage_done[k]));  // its last copy has run
+    *img = sc->stage[k];
+    if (params_changed(sc)) {
+        code_desc* d = reinterpret_cast<code_desc*>(sc->stage[k] + sc->plan_bytes);
+        fill_code_descs(sc, d);
+        HIP_TRY(hipMemcpyAsync(sc->d_codes, d, sc->codes.size() * sizeof(code_desc), hipMemcpyHostToDevice, st));
+        note_params(sc);
+    }
+    return CUSAT_OK;
+}
+
+// The staging image taken by next_stage_image has been copied on `st`.
+int release_stage_image(cusat_stream_ctx* sc, hipStream_t st)
+{
+    const int k = sc->next_stage;
+    HIP_TRY(hipEventRecord(sc->stage_done[k], st));
+    sc->stage_used[k] = true;
+    sc->next_stage = (k + 1) % cusat_stream_ctx::N_STAGE;
     return CUSAT_OK;
 }
 
@@ -663,29 +720,48 @@ extern "C" int cusat_stream_create(cusat_stream_ctx** out, cusat_fec_ctx* const*
     sc->off_ev = sc->off_rx + 256 + al(sizeof(rx_rec) * max_frames);
     sc->plan_bytes = sc->off_ev + al(sizeof(rx_event) * sc->max_events);
     const size_t bytes = sc->plan_bytes + al(sizeof(code_desc) * n_codes) + al(sizeof(rx_code) * n_codes);
+    sc->stage_bytes = sc->plan_bytes + sizeof(code_desc) * n_codes;
+    for (const cusat_fec_ctx* c : sc->codes) {
+        sc->llr_max = std::max<int64_t>(sc->llr_max, c->geo.N);
+        sc->hard_max = std::max<int64_t>(sc->hard_max, c->geo.K / 8);
+    }
+    sc->code_gen.assign(n_codes, 0);
+    // every device and pinned buffer of the context is allocated here, so
+    // that no later call allocates (the async plan calls are capturable and
+    // cheap); destroy frees whatever part was allocated
     char* base = nullptr;
     e = hipMalloc(&base, bytes);
+    if (e == hipSuccess) sc->d_counters = reinterpret_cast<unsigned*>(base);
     if (e == hipSuccess) e = hipMalloc(&sc->d_ws, n_wg * rec_max);
+    if (e == hipSuccess) e = hipMalloc(&sc->d_llr_work, (size_t)max_frames * sc->llr_max * sizeof(int16_t));
+    if (e == hipSuccess) e = hipMalloc(&sc->d_hard_work, (size_t)max_frames * sc->hard_max);
+    if (e == hipSuccess) e = hipMalloc(&sc->d_snap, n_codes * sizeof(rx_scale));
+    if (e == hipSuccess) e = hipMalloc(&sc->d_est, sc->max_events * sizeof(float));
+    for (int k = 0; k < cusat_stream_ctx::N_STAGE && e == hipSuccess; k++) {
+        e = hipHostMalloc((void**)&sc->stage[k], sc->stage_bytes, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sc->stage_done[k], hipEventDisableTiming);
+    }
     if (e != hipSuccess) {
-        if (base) hipFree(base);
-        delete sc;
         set_last_error("stream buffers: %s", hipGetErrorString(e));
+        cusat_stream_destroy(sc);
         return CUSAT_E_NOMEM;
     }
-    sc->d_counters = reinterpret_cast<unsigned*>(base);
     sc->d_frames = reinterpret_cast<cusat_stream_frame*>(base + 256);
     sc->d_jobs = reinterpret_cast<int2*>(base + sc->off_jobs);
     sc->d_order = reinterpret_cast<int*>(base + sc->off_order);
     sc->d_codes = reinterpret_cast<code_desc*>(base + sc->plan_bytes);
     sc->d_rx_codes = reinterpret_cast<rx_code*>(base + sc->plan_bytes + al(sizeof(code_desc) * n_codes));
-    for (const cusat_fec_ctx* c : sc->codes) {
-        sc->llr_max = std::max<int64_t>(sc->llr_max, c->geo.N);
-        sc->hard_max = std::max<int64_t>(sc->hard_max, c->geo.K / 8);
-    }
-    if (hipMemset(sc->d_counters, 0, 256) != hipSuccess) {
+    if (hipMemset(sc->d_counters, 0, 256) != hipSuccess || hipMemset(sc->d_snap, 0, n_codes * sizeof(rx_scale)) !=
+                                                                  hipSuccess) {
         cusat_stream_destroy(sc);
         set_last_error("counter init failed");
         return CUSAT_E_HIP;
+    }
+    const int urc = upload_codes(sc, true);
+    if (urc != CUSAT_OK || hipStreamSynchronize(nullptr) != hipSuccess) {
+        cusat_stream_destroy(sc);
+        if (urc == CUSAT_OK) set_last_error("stream set-up: %s", hipGetErrorString(hipGetLastError()));
+        return urc != CUSAT_OK ? urc : CUSAT_E_HIP;
     }
     // the 12-wave kernel needs a split variant of every code
     if (std::any_of(sc->variant_split.begin(), sc->variant_split.end(), [](int v) { return v < 0; }))
@@ -741,9 +817,8 @@ extern "C" int cusat_stream_plan(cusat_stream_ctx* sc, const cusat_stream_frame*
         }
     sc->frames.assign(frames, frames + n_frames);
     sc->n_frames = n_frames;
-    int rc = upload_codes(sc);
+    int rc = upload_codes(sc, false);
     if (rc != CUSAT_OK) return rc;
-    sc->codes_uploaded = true;
     sc->rx_planned = false;
     return build_plan(sc, tu);
 }
@@ -761,25 +836,16 @@ extern "C" int cusat_stream_plan_async(cusat_stream_ctx* sc, const cusat_stream_
             set_last_error("frame %d: code %d / offsets out of range", k, frames[k].code);
             return CUSAT_E_INVALID;
         }
-    if (!sc->codes_uploaded) {
-        const int rc = upload_codes(sc);
-        if (rc != CUSAT_OK) return rc;
-        sc->codes_uploaded = true;
-    }
-    const int k = sc->next_stage;
-    if (!sc->stage[k]) {
-        HIP_TRY(hipHostMalloc((void**)&sc->stage[k], sc->plan_bytes, hipHostMallocDefault));
-        HIP_TRY(hipEventCreateWithFlags(&sc->stage_done[k], hipEventDisableTiming));
-    }
-    if (sc->stage_used[k]) HIP_TRY(hipEventSynchronize(sc->stage_done[k]));  // its last copy has run
+    hipStream_t st = (hipStream_t)stream;
+    unsigned char* img = nullptr;
+    int rc = next_stage_image(sc, st, &img);
+    if (rc != CUSAT_OK) return rc;
     sc->frames.assign(frames, frames + n_frames);
     sc->n_frames = n_frames;
-    const int nj = stage_plan(sc, sc->tuning, sc->stage[k]);
-    hipStream_t st = (hipStream_t)stream;
-    HIP_TRY(copy_plan(sc, sc->stage[k], n_frames, nj, st, true));
-    HIP_TRY(hipEventRecord(sc->stage_done[k], st));
-    sc->stage_used[k] = true;
-    sc->next_stage = (k + 1) % cusat_stream_ctx::N_STAGE;
+    const int nj = stage_plan(sc, sc->tuning, img);
+    HIP_TRY(copy_plan(sc, img, n_frames, nj, st, true));
+    rc = release_stage_image(sc, st);
+    if (rc != CUSAT_OK) return rc;
     sc->n_jobs = nj;
     sc->rx_planned = false;
     return CUSAT_OK;
@@ -860,25 +926,10 @@ extern "C" int cusat_stream_rx_plan_async(cusat_stream_ctx* sc, const cusat_stre
             return CUSAT_E_INVALID;
         }
     }
-    if (!sc->codes_uploaded) {
-        const int rc = upload_codes(sc);
-        if (rc != CUSAT_OK) return rc;
-        sc->codes_uploaded = true;
-    }
-    if (!sc->d_llr_work) {
-        HIP_TRY(hipMalloc(&sc->d_llr_work, (size_t)sc->max_frames * sc->llr_max * sizeof(int16_t)));
-        HIP_TRY(hipMalloc(&sc->d_hard_work, (size_t)sc->max_frames * sc->hard_max));
-        HIP_TRY(hipMalloc(&sc->d_snap, sc->codes.size() * sizeof(rx_scale)));
-        HIP_TRY(hipMalloc(&sc->d_est, sc->max_events * sizeof(float)));
-        HIP_TRY(hipMemset(sc->d_snap, 0, sc->codes.size() * sizeof(rx_scale)));
-    }
-    const int k = sc->next_stage;
-    if (!sc->stage[k]) {
-        HIP_TRY(hipHostMalloc((void**)&sc->stage[k], sc->plan_bytes, hipHostMallocDefault));
-        HIP_TRY(hipEventCreateWithFlags(&sc->stage_done[k], hipEventDisableTiming));
-    }
-    if (sc->stage_used[k]) HIP_TRY(hipEventSynchronize(sc->stage_done[k]));  // its last copy has run
-    unsigned char* img = sc->stage[k];
+    hipStream_t st = (hipStream_t)stream;
+    unsigned char* img = nullptr;
+    int rc = next_stage_image(sc, st, &img);
+    if (rc != CUSAT_OK) return rc;
     int* hdr = reinterpret_cast<int*>(img + sc->off_rx);
     rx_rec* rec = reinterpret_cast<rx_rec*>(img + sc->off_rx + 256);
     // the decoder's frames: LLRs and hard decisions packed in the work buffers
@@ -914,7 +965,6 @@ extern "C" int cusat_stream_rx_plan_async(cusat_stream_ctx* sc, const cusat_stre
         ev[e] = rx_event{events[e].code, events[e].frame, events[e].snr_lin, events[e].snr_db};
     sc->n_frames = n_frames;
     const int nj = stage_plan(sc, sc->tuning, img);
-    hipStream_t st = (hipStream_t)stream;
     unsigned char* d = reinterpret_cast<unsigned char*>(sc->d_counters);
     const size_t span = sc->off_ev + n_events * sizeof(rx_event);
     if (span <= (256u << 10)) {
@@ -929,11 +979,63 @@ extern "C" int cusat_stream_rx_plan_async(cusat_stream_ctx* sc, const cusat_stre
             HIP_TRY(hipMemcpyAsync(d + sc->off_ev, img + sc->off_ev, n_events * sizeof(rx_event),
                                    hipMemcpyHostToDevice, st));
     }
-    HIP_TRY(hipEventRecord(sc->stage_done[k], st));
-    sc->stage_used[k] = true;
-    sc->next_stage = (k + 1) % cusat_stream_ctx::N_STAGE;
+    rc = release_stage_image(sc, st);
+    if (rc != CUSAT_OK) return rc;
     sc->n_jobs = nj;
     sc->rx_planned = true;
+    return CUSAT_OK;
+}
+
+namespace {
+
+int need_rx_plan(const cusat_stream_ctx* sc, const char* what)
+{
+    if (!sc->rx_planned) {
+        set_last_error("%s needs an rx plan (cusat_stream_rx_plan_async)", what);
+        return CUSAT_E_INVALID;
+    }
+    return CUSAT_OK;
+}
+
+const int* rx_header(const cusat_stream_ctx* sc)
+{
+    return reinterpret_cast<const int*>(reinterpret_cast<const unsigned char*>(sc->d_counters) + sc->off_rx);
+}
+
+}  // namespace
+
+extern "C" int cusat_stream_rx_demap(cusat_stream_ctx* sc, const float* sym, void* stream)
+{
+    if (!sc || !sym) { set_last_error("bad argument"); return CUSAT_E_INVALID; }
+    const int rc = need_rx_plan(sc, "cusat_stream_rx_demap");
+    if (rc != CUSAT_OK) return rc;
+    const int* hdr = rx_header(sc);
+    const rx_rec* rec = reinterpret_cast<const rx_rec*>(hdr + 64);
+    HIP_TRY(launch_demap_mixed(sc->d_rx_codes, rec, sc->d_snap, hdr, sym, sc->d_llr_work, sc->n_cu * 4,
+                               (hipStream_t)stream));
+    return CUSAT_OK;
+}
+
+extern "C" int cusat_stream_rx_decode(cusat_stream_ctx* sc, int32_t* trials, int32_t* valid, int32_t* fallback,
+                                      void* stream)
+{
+    if (!sc || !trials || !valid) { set_last_error("bad argument"); return CUSAT_E_INVALID; }
+    const int rc = need_rx_plan(sc, "cusat_stream_rx_decode");
+    if (rc != CUSAT_OK) return rc;
+    HIP_TRY(launch_stream(sc, sc->d_llr_work, sc->d_hard_work, trials, valid, fallback, stream));
+    return CUSAT_OK;
+}
+
+extern "C" int cusat_stream_rx_bb(cusat_stream_ctx* sc, uint8_t* out, const int32_t* trials, const int32_t* valid,
+                                  int32_t* dfl, void* stream)
+{
+    if (!sc || !out || !trials || !valid || !dfl) { set_last_error("bad argument"); return CUSAT_E_INVALID; }
+    const int rc = need_rx_plan(sc, "cusat_stream_rx_bb");
+    if (rc != CUSAT_OK) return rc;
+    const int* hdr = rx_header(sc);
+    const rx_rec* rec = reinterpret_cast<const rx_rec*>(hdr + 64);
+    HIP_TRY(launch_bb_extract_mixed(sc->d_rx_codes, rec, hdr, sc->d_hard_work, trials, valid, out, dfl, sc->n_cu * 2,
+                                    (hipStream_t)stream));
     return CUSAT_OK;
 }
 
@@ -941,17 +1043,17 @@ extern "C" int cusat_stream_rx(cusat_stream_ctx* sc, const float* sym, uint8_t* 
                                int32_t* valid, int32_t* dfl, void* stream)
 {
     if (!sc || !sym || !out || !trials || !valid || !dfl) { set_last_error("bad argument"); return CUSAT_E_INVALID; }
-    if (!sc->rx_planned) {
-        set_last_error("cusat_stream_rx needs an rx plan (cusat_stream_rx_plan_async)");
-        return CUSAT_E_INVALID;
-    }
-    hipStream_t st = (hipStream_t)stream;
-    const int* hdr = reinterpret_cast<const int*>(reinterpret_cast<unsigned char*>(sc->d_counters) + sc->off_rx);
-    const rx_rec* rec = reinterpret_cast<const rx_rec*>(hdr + 64);
-    HIP_TRY(launch_demap_mixed(sc->d_rx_codes, rec, sc->d_snap, hdr, sym, sc->d_llr_work, sc->n_cu * 4, st));
-    HIP_TRY(launch_stream(sc, sc->d_llr_work, sc->d_hard_work, trials, valid, nullptr, stream));
-    HIP_TRY(launch_bb_extract_mixed(sc->d_rx_codes, rec, hdr, sc->d_hard_work, trials, valid, out, dfl, sc->n_cu * 2,
-                                    st));
+    int rc = cusat_stream_rx_demap(sc, sym, stream);
+    if (rc == CUSAT_OK) rc = cusat_stream_rx_decode(sc, trials, valid, nullptr, stream);
+    if (rc == CUSAT_OK) rc = cusat_stream_rx_bb(sc, out, trials, valid, dfl, stream);
+    return rc;
+}
+
+extern "C" int cusat_stream_rx_work(const cusat_stream_ctx* sc, const int16_t** llr, const uint8_t** hard)
+{
+    if (!sc) { set_last_error("bad argument"); return CUSAT_E_INVALID; }
+    if (llr) *llr = sc->d_llr_work;
+    if (hard) *hard = sc->d_hard_work;
     return CUSAT_OK;
 }
 
@@ -959,10 +1061,8 @@ extern "C" int cusat_stream_rx_dd(cusat_stream_ctx* sc, const float* sym, float*
                                   void* stream)
 {
     if (!sc || !sym || !state) { set_last_error("bad argument"); return CUSAT_E_INVALID; }
-    if (!sc->rx_planned) {
-        set_last_error("cusat_stream_rx_dd needs an rx plan (cusat_stream_rx_plan_async)");
-        return CUSAT_E_INVALID;
-    }
+    const int rc = need_rx_plan(sc, "cusat_stream_rx_dd");
+    if (rc != CUSAT_OK) return rc;
     hipStream_t st = (hipStream_t)stream;
     const unsigned char* base = reinterpret_cast<const unsigned char*>(sc->d_counters);
     const int* hdr = reinterpret_cast<const int*>(base + sc->off_rx);

@@ -56,6 +56,133 @@ def make_stream(workload, n_frames, seed, log=print):
     return out
 
 
+_POPCOUNT = np.array([bin(v).count("1") for v in range(256)], np.uint8)
+
+
+def bbheader(kbch_bytes):
+    """BBHEADER (EN 302 307-1 clause 5.1.6, table 3) of a frame whose data
+    field fills the BBFRAME: MATYPE-1 = generic packetized TS/GS, single
+    input stream, CCM, roll-off 0.20; UPL 0, DFL = 8 (k_bch/8 - 10) bits,
+    SYNC 0, SYNCD 0xFFFF, then the CRC-8 of the first nine bytes."""
+    h = np.zeros(10, np.uint8)
+    dfl = 8 * (kbch_bytes - 10)
+    h[0] = (1 << 5) | (1 << 4) | 2
+    h[4], h[5] = dfl >> 8, dfl & 0xFF
+    h[7] = h[8] = 0xFF
+    import cusat_amd as ca
+    h[9] = ca.bb_crc8(h[:9])
+    return h
+
+
+class RxStream:
+    """A mixed ACM stream for the receive path (cusat_stream_rx): frames of
+    make_stream's composition, each a scrambled BBFRAME (bbheader + a
+    data field of seeded random bytes) through the GPU transmit chain
+    (cusat_gen_frames_dev: BCH, LDPC, interleave + map, AWGN at the frame's
+    Es/N0), symbols packed back to back in arrival order in HBM.  Holds the
+    sent data fields and codewords to score what the receiver delivers."""
+
+    def __init__(self, workload="s2x", n_frames=None, seed=20240612, learned=True, max_trials=50):
+        import torch
+        import cusat_amd as ca
+        from cusat_amd import learn
+        n_frames = n_frames or (2048 if workload == "mixed12" else 4096)
+        spec = make_stream(workload, n_frames, seed)
+        self.spec = spec
+        self.keys = sorted(set(spec))
+        idx = {k: i for i, k in enumerate(self.keys)}
+        count = {k: 0 for k in self.keys}
+        for s_ in spec:
+            count[s_] += 1
+        g = torch.Generator(device="cuda")
+        g.manual_seed(seed)
+        self.ctxs, self.learned, self.params = [], set(), {}
+        sym_g, pay_g, cw_g = [], [], []
+        for gi, k in enumerate(self.keys):
+            fs, rate, mod, es = k
+            c = ca.FecContext(fs, rate, mod, max_trials=max_trials, max_batch=count[k])
+            inf = c.info
+            kb = inf["k_bch"] // 8
+            B = count[k]
+            hdr = torch.from_numpy(bbheader(kb)).cuda()
+            prbs = torch.from_numpy(ca.bb_prbs(kb)).cuda()
+            pay = torch.randint(0, 256, (B, kb - 10), generator=g, device="cuda", dtype=torch.uint8)
+            frame = torch.cat([hdr.expand(B, 10), pay], 1) ^ prbs
+            sym = torch.empty((B, inf["xfec_len"], 2), dtype=torch.float32, device="cuda")
+            cw = torch.empty((B, inf["n_ldpc"] // 8), dtype=torch.uint8, device="cuda")
+            c.gen_frames_dev(seed * 131 + gi, B, es, sym, info_in=frame.contiguous(), cw_out=cw)
+            if learned:
+                path = learn.params_for(fs, rate, mod)
+                if path:
+                    _, bq, aq = learn.load_params(path)
+                    c.set_params(bq, aq)
+                    self.learned.add(gi)
+                    self.params[gi] = (bq, aq)
+            self.ctxs.append(c)
+            sym_g.append(sym)
+            pay_g.append(pay)
+            cw_g.append(cw)
+        # arrival order, offsets, per-frame constants
+        slot = {k: 0 for k in self.keys}
+        n = len(spec)
+        self.n = n
+        self.code = np.zeros(n, np.int32)
+        self.row = np.zeros(n, np.int32)
+        for f, s_ in enumerate(spec):
+            self.code[f], self.row[f] = idx[s_], slot[s_]
+            slot[s_] += 1
+        info = [c.info for c in self.ctxs]
+        self.N = np.array([info[c]["n_ldpc"] for c in self.code], np.int64)
+        self.K = np.array([info[c]["k_ldpc"] for c in self.code], np.int64)
+        self.kbch = np.array([info[c]["k_bch"] for c in self.code], np.int64)
+        self.t_bch = np.array([info[c]["bch_t"] for c in self.code], np.int64)
+        self.lt = np.array([info[c]["lt_alloc"] for c in self.code], np.int64)
+        xf = np.array([info[c]["xfec_len"] for c in self.code], np.int64)
+        self.sym_off = np.concatenate([[0], np.cumsum(xf)[:-1]])
+        self.dlen = self.kbch // 8 - 10
+        self.out_off = np.concatenate([[0], np.cumsum(self.dlen)[:-1]])
+        self.llr_off = np.concatenate([[0], np.cumsum(self.N)[:-1]])
+        self.hard_off = np.concatenate([[0], np.cumsum(self.K // 8)[:-1]])
+        self.out_bytes = int(self.dlen.sum())
+        self.sym = torch.empty((int(xf.sum()), 2), dtype=torch.float32, device="cuda")
+        sent = torch.empty(self.out_bytes, dtype=torch.uint8, device="cuda")
+        cw_all = torch.empty(int((self.K // 8).sum()), dtype=torch.uint8, device="cuda")
+        for f in range(n):
+            c, j = self.code[f], self.row[f]
+            so, x = int(self.sym_off[f]), int(xf[f])
+            self.sym[so:so + x] = sym_g[c][j]
+            oo, d = int(self.out_off[f]), int(self.dlen[f])
+            sent[oo:oo + d] = pay_g[c][j]
+            ho, kb8 = int(self.hard_off[f]), int(self.K[f] // 8)
+            cw_all[ho:ho + kb8] = cw_g[c][j, :kb8]
+        torch.cuda.synchronize()
+        self.sent = sent.cpu().numpy()
+        self.cw = cw_all.cpu().numpy()  # the BCH codewords (= systematic LDPC info part)
+        self.recs = np.zeros(n, ca.STREAM_RX_FRAME)
+        for f in range(n):
+            nv, gain = snr_scales(self.keys[self.code[f]][3])
+            self.recs[f] = (self.code[f], 0, self.sym_off[f], self.out_off[f], nv, gain)
+        # learned iterations per frame (0: the code decodes plain)
+        self.t_learned = np.array([self.params[c][0].shape[0] if c in self.params else 0 for c in self.code],
+                                  np.int64)
+        self.groups = len(self.keys)
+        self.modcods = len(set((k[1], k[2]) for k in self.keys))
+        self.short_fraction = float(np.mean([s_[0] == 1 for s_ in spec]))
+
+    def score(self, trials, valid, dfl, out, hard=None):
+        """Per-frame delivery (data field equal to the one sent) and, given
+        the hard decisions (the work buffer, plan layout), the bit errors of
+        the decoded BCH codeword against the sent one."""
+        out = np.asarray(out)[:self.out_bytes]
+        bad = np.add.reduceat(out != self.sent, self.out_off, dtype=np.int64)
+        ok = (np.asarray(dfl) == self.dlen) & (bad == 0) & (np.asarray(trials) >= 0) & (np.asarray(valid) == 1)
+        res = {"ok": ok}
+        if hard is not None:
+            x = _POPCOUNT[np.bitwise_xor(np.asarray(hard)[:self.cw.size], self.cw)]
+            res["bit_errors"] = np.add.reduceat(x, self.hard_off, dtype=np.int64)
+        return res
+
+
 def run(workload="mixed12", n_frames=None, seed=20240612, reps=5, learned=None, log=print):
     import torch
     import cusat_amd as ca

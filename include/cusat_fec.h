@@ -133,8 +133,15 @@ int cusat_rx_fec_batch(cusat_fec_ctx* ctx, const float* sym_dev, int16_t* llr_de
  * cusat_ldpc_set_params: beta[t*q + i] >= 0 (LLR units) and alpha[t*q + i]
  * in 0..1023 (n/64; 0 and 64 leave the message unscaled) for iterations
  * t < n_iters and layers i < q; iterations t >= n_iters run the plain check
- * node.  n_iters = 0 clears them.  Not to be called while a decode on this
- * context is in flight. */
+ * node.  n_iters = 0 clears them.  Not to be called while a decode that uses
+ * this context (its own, or a stream context's) is in flight.  A stream
+ * context holding this context takes the new iteration count and table
+ * address at its next cusat_stream_plan / cusat_stream_plan_async /
+ * cusat_stream_rx_plan_async (the async calls copy them on their stream,
+ * ahead of the plan; a captured decode graph replayed after that call uses
+ * them), so plan a stream again before its next decode.  A table outgrown
+ * here stays allocated until the context is destroyed, so no launch reads
+ * freed memory. */
 int cusat_ldpc_set_params(cusat_fec_ctx* ctx, const int16_t* beta, const int16_t* alpha, int n_iters);
 /* cusat_ldpc_decode with the parameters above and max_trials iterations
  * (replaces decode_launch_async with the hooks set). */
@@ -294,8 +301,8 @@ typedef struct cusat_stream_rx_event {
  * `stream` like cusat_stream_plan_async), with n_events SNR events (at most
  * 2 max_frames + the number of codes).  LLRs and hard decisions live in
  * work buffers of the stream context (max_frames x the largest code,
- * allocated by the first call).  A later cusat_stream_plan /
- * cusat_stream_plan_async replaces it. */
+ * allocated by cusat_stream_create: this call allocates nothing).  A later
+ * cusat_stream_plan / cusat_stream_plan_async replaces it. */
 int cusat_stream_rx_plan_async(cusat_stream_ctx* sc, const cusat_stream_rx_frame* frames, int n_frames,
                                const cusat_stream_rx_event* events, int n_events, void* stream);
 /* Apply the current batch's events to state (device, float4 per code) and
@@ -309,6 +316,22 @@ int cusat_stream_rx_dd(cusat_stream_ctx* sc, const float* sym, float* state, flo
  * cusat_bb_extract writes it: -1 FEC failure, 0 no data field, n bytes). */
 int cusat_stream_rx(cusat_stream_ctx* sc, const float* sym, uint8_t* out, int32_t* trials, int32_t* valid,
                     int32_t* dfl, void* stream);
+/* The three launches of cusat_stream_rx one at a time, in this order on one
+ * stream (for a caller that times or overlaps them): the mixed demap into
+ * the LLR work buffer; the persistent decode + BCH into the hard-decision
+ * work buffer (fallback: B ints or NULL, 1 = redone by the plain decoder in
+ * CUSAT_STREAM_LEARNED mode); the BB step.  Same arguments as
+ * cusat_stream_rx. */
+int cusat_stream_rx_demap(cusat_stream_ctx* sc, const float* sym, void* stream);
+int cusat_stream_rx_decode(cusat_stream_ctx* sc, int32_t* trials, int32_t* valid, int32_t* fallback,
+                           void* stream);
+int cusat_stream_rx_bb(cusat_stream_ctx* sc, uint8_t* out, const int32_t* trials, const int32_t* valid,
+                       int32_t* dfl, void* stream);
+/* Device addresses of the rx work buffers (either pointer may be NULL): the
+ * current rx plan packs frame f's LLRs at the sum of n_ldpc over the frames
+ * before it, and its hard decisions (k_ldpc/8 bytes) at the sum of k_ldpc/8.
+ * Valid for the life of the stream context. */
+int cusat_stream_rx_work(const cusat_stream_ctx* sc, const int16_t** llr, const uint8_t** hard);
 /* Decode the planned frames: llr (device int16), hard (device bytes),
  * trials / valid / fallback (B ints each, fallback may be NULL) indexed by
  * frame number. */

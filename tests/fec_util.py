@@ -5,24 +5,7 @@ import numpy as np
 import cusat_amd as ca
 from oracle import oracle_ctypes as oc
 
-TABLE = {  # (framesize, rate) -> oracle table name
-    (0, "1/4"): "s2_b1", (0, "1/3"): "s2_b2", (0, "2/5"): "s2_b3", (0, "1/2"): "s2_b4",
-    (0, "3/5"): "s2_b5", (0, "2/3"): "s2_b6", (0, "3/4"): "s2_b7", (0, "4/5"): "s2_b8",
-    (0, "5/6"): "s2_b9", (0, "8/9"): "s2_b10", (0, "9/10"): "s2_b11",
-    (0, "135/180"): "s2x_b19", (0, "128/180"): "s2x_b17", (0, "154/180"): "s2x_b21",
-    (0, "96/180"): "s2x_b12", (0, "22/30"): "s2x_b24", (0, "7/9"): "s2x_b10", (1, "7/15"): "s2x_c4",
-    (1, "1/4"): "s2_c1", (1, "1/3"): "s2_c2", (1, "2/5"): "s2_c3", (1, "1/2"): "s2_c4",
-    (1, "3/5"): "s2_c5", (1, "2/3"): "s2_c6", (1, "3/4"): "s2_c7", (1, "4/5"): "s2_c8",
-    (1, "5/6"): "s2_c9", (1, "8/9"): "s2_c10",
-    # the other DVB-S2X codes, by their table in EN 302 307-2 annexes B and C
-    (0, "13/45"): "s2x_b2", (0, "9/20"): "s2x_b3", (0, "11/20"): "s2x_b4", (0, "26/45"): "s2x_b5",
-    (0, "28/45"): "s2x_b6", (0, "23/36"): "s2x_b7", (0, "25/36"): "s2x_b8", (0, "13/18"): "s2x_b9",
-    (0, "90/180"): "s2x_b11", (0, "100/180"): "s2x_b13", (0, "104/180"): "s2x_b14",
-    (0, "116/180"): "s2x_b15", (0, "124/180"): "s2x_b16", (0, "132/180"): "s2x_b18",
-    (0, "140/180"): "s2x_b20", (0, "18/30"): "s2x_b22", (0, "20/30"): "s2x_b23",
-    (1, "11/45"): "s2x_c1", (1, "4/15"): "s2x_c2", (1, "14/45"): "s2x_c3", (1, "8/15"): "s2x_c5",
-    (1, "26/45"): "s2x_c6", (1, "32/45"): "s2x_c7",
-}
+TABLE = oc.TABLE
 
 _codes = {}
 

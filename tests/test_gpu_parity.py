@@ -76,6 +76,22 @@ def test_s2x_codes_with_uneven_layers(fs, rate, mod, es):
     _chain_parity(fs, rate, mod, es, 12, 21, flags=ca.CTX_NO_SPLIT_KERNEL)
 
 
+@pytest.mark.parametrize("fs,rate,mod,es", [(0, "4/5", "64APSK", 16.4), (0, "135/180", "256APSK", 23.6),
+                                             (0, "132/180", "64APSK", 15.3), (0, "100/180", "8APSK", 5.2),
+                                             (0, "20/30", "256APSK", 18.4), (0, "22/30", "256APSK", 20.1),
+                                             (0, "128/180", "256APSK", 22.6)])
+def test_s2x_apsk_rings(fs, rate, mod, es):
+    """DVB-S2X constellations of the combined stream and their neighbours,
+    at threshold + 0.3 dB (acm_config.h:68,109-121): 64APSK 4/5 (8+16+20+20
+    rings, constellation.cc:310-318) and 11/15 = 132/180 (4+12+20+28), 8APSK 5/9,
+    256APSK 3/4 and 32/45 (ring form, constellation.cc:400) and the 256APSK-L
+    rates 2/3 and 11/15 (explicit Table 15d points); n_mod 6 and 8 with 6 and
+    8 interleaver columns.  LLRs of the HIP demapper bit-exact vs the oracle,
+    then the decode and BCH flags."""
+    tr, va, _, _ = _chain_parity(fs, rate, mod, es, 12, 61)
+    assert (tr >= 0).sum() >= 6, "near threshold most frames converge"
+
+
 def test_sequential_schedule_flag():
     """CUSAT_CTX_SEQUENTIAL_SCHEDULE = the reference's CUSAT_LDPC_SEQ=1."""
     _chain_parity(0, "3/4", "8PSK", 8.0, 16, 3, flags=ca.CTX_SEQUENTIAL_SCHEDULE)
