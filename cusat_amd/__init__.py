@@ -74,6 +74,7 @@ def lib():
         L.cusat_fec_constellation.argtypes = [vp, vp, vp]
         L.cusat_fec_demap.argtypes = [vp, vp, vp, i, f, f, vp]
         L.cusat_ldpc_decode.argtypes = [vp, vp, vp, vp, i, vp]
+        L.cusat_ldpc_decode_noalloc.argtypes = [vp, vp, vp, vp, i, vp]
         L.cusat_bch_check.argtypes = [vp, vp, vp, i, vp]
         L.cusat_rx_fec_batch.argtypes = [vp, vp, vp, i, f, f, vp, vp, vp, vp]
         L.cusat_gen_frames.argtypes = [i, i, i, ctypes.c_uint64, i, f, vp, vp, vp]

@@ -309,6 +309,9 @@ extern "C" int cusat_fec_ctx_create(cusat_fec_ctx** out, int framesize, int rate
         return rc;
     }
     hipGetDevice(&c->device);
+    // set-up of cusat_ldpc_decode_noalloc on this device (the address of its
+    // static scratch); a failure here shows again, as an error, in that call
+    (void)ldpc_scratch_setup();
     rc = upload_tables(c);
     if (rc == CUSAT_OK) {
         const size_t rec = std::max(ldpc_msg_bytes(c->geo.q, c->cmax), ldpc_msg_bytes_split(c->geo.q, c->cmax));

@@ -41,6 +41,15 @@ bool ldpc_select_variant(int cnt_min, int cnt_max, int lanes, int* cmax, bool* e
 // message record bytes per frame for a variant
 size_t ldpc_msg_bytes(int q, int cmax);
 hipError_t launch_ldpc_decode(const ldpc_args& a, int batch, hipStream_t stream);
+}  // namespace cusat
+struct cusat_fec_ctx;
+namespace cusat {
+// the decode arguments of a context (tables, geometry, its message
+// workspace) for the given LLR and output buffers
+ldpc_args make_ldpc_args(const cusat_fec_ctx* c, const int16_t* llr, uint8_t* hard, int32_t* trials);
+// look up (once per device, with the device current) the address of the
+// decoder's static message scratch; 0 on success
+int ldpc_scratch_setup();
 // split-edge kernel: exists for every variant; its LDS adds the partial
 // exchange area, its message records are [layer][half][pair]
 bool ldpc_split_available(int cmax, bool exact, int lanes);

@@ -1,5 +1,10 @@
 // LDPC decode: variant selection and dispatch (kernel in ldpc_kernel.h,
-// instantiations in ldpc_inst_*.hip).
+// instantiations in ldpc_inst_*.hip), and the decode entry point that
+// allocates nothing (message records in a static device array).
+#include <mutex>
+
+#include "ctx.h"
+#include "cusat_fec.h"
 #include "kernels.h"
 
 namespace cusat {
@@ -102,3 +107,99 @@ hipError_t launch_ldpc_decode(const ldpc_args& a, int batch, hipStream_t stream)
 }
 
 }  // namespace cusat
+
+// ---- cusat_ldpc_decode_noalloc: message records in a static device array ----
+//
+// The decoder's only scratch is the check-to-variable message records of the
+// frames in flight (one record set per frame of the batch; the kernel writes
+// every record before it reads it, so nothing needs clearing).  This entry
+// point keeps them in msg_scratch, a file-scope device array that every GPU
+// the library runs on holds (its own copy) until the process ends, instead
+// of the context's workspace.  In host code the array's name is not a device
+// address: the address of the current device's copy comes from
+// hipGetSymbolAddress, looked up once per device and used everywhere the
+// host hands the array to a kernel.  All calls on one GPU share its copy, so
+// calls that overlap on one GPU (two streams, two threads) race.
+
+namespace cusat {
+
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+
+static __device__ __align__(256) unsigned char msg_scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// device address of the current device's msg_scratch, looked up on first use
+int scratch_address(void** out)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+        set_last_error("message scratch: no current device (%s)", hipGetErrorString(e));
+        return CUSAT_E_NODEVICE;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(msg_scratch));
+        if (e != hipSuccess || !p) {
+            set_last_error("message scratch: hipGetSymbolAddress: %s", hipGetErrorString(e));
+            return CUSAT_E_HIP;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return CUSAT_OK;
+}
+
+}  // namespace
+
+int ldpc_scratch_setup()
+{
+    void* p = nullptr;
+    return scratch_address(&p);
+}
+
+}  // namespace cusat
+
+using namespace cusat;
+
+extern "C" int cusat_ldpc_decode_noalloc(cusat_fec_ctx* c, const int16_t* llr, uint8_t* hard, int32_t* trials,
+                                         int batch, void* stream)
+{
+    if (!c || !llr || !hard || !trials) {
+        set_last_error("bad argument");
+        return CUSAT_E_INVALID;
+    }
+    const bool split = !(c->flags & CUSAT_CTX_NO_SPLIT_KERNEL);
+    // the records of the kernel the dispatch will pick (the split kernel's
+    // layout where it is used, else the 6-wave kernel's)
+    const size_t rec = std::max(ldpc_msg_bytes(c->geo.q, c->cmax),
+                                split ? ldpc_msg_bytes_split(c->geo.q, c->cmax) : (size_t)0);
+    const long max_batch = (long)(NOALLOC_SCRATCH_BYTES / rec);
+    if (batch < 1 || batch > max_batch) {
+        set_last_error("batch %d outside 1..%ld (%zu bytes of message records per frame, %zu-byte static scratch)",
+                       batch, max_batch, rec, NOALLOC_SCRATCH_BYTES);
+        return CUSAT_E_BATCH;
+    }
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev != c->device) {
+        set_last_error("context of device %d used on device %d", c->device, dev);
+        return CUSAT_E_INVALID;
+    }
+    void* scratch = nullptr;
+    const int rc = scratch_address(&scratch);
+    if (rc != CUSAT_OK) return rc;
+    ldpc_args a = make_ldpc_args(c, llr, hard, trials);
+    a.msg = static_cast<int8_t*>(scratch);
+    const hipError_t e = launch_ldpc_decode(a, batch, (hipStream_t)stream);
+    if (e != hipSuccess) {
+        set_last_error("cusat_ldpc_decode_noalloc launch: %s", hipGetErrorString(e));
+        return CUSAT_E_HIP;
+    }
+    return CUSAT_OK;
+}

@@ -111,6 +111,20 @@ int cusat_fec_demap(cusat_fec_ctx* ctx, const float* sym_dev, int16_t* llr_dev, 
 int cusat_ldpc_decode(cusat_fec_ctx* ctx, const int16_t* llr_dev, uint8_t* hard_out,
                       int32_t* trials_out, int batch, void* stream);
 
+/* No counterpart in the reference: cusat_ldpc_decode for callers that
+ * cannot allocate device memory (code under hipGraph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and results as
+ * cusat_ldpc_decode, but the message records live in a static device array
+ * of 64 MiB per GPU (held from the library's load until the process ends),
+ * not in the context's workspace, so the batch is bounded by that array
+ * (batch x the code's message bytes per frame <= 64 MiB: at least 86
+ * normal frames of any code, 258 of 8PSK 3/4), not by max_batch; larger
+ * batches return CUSAT_E_BATCH.  The call allocates nothing.  All calls on one GPU share
+ * its array: calls that overlap on one GPU (other streams or threads) race.
+ * Call it with the context's device current. */
+int cusat_ldpc_decode_noalloc(cusat_fec_ctx* ctx, const int16_t* llr_dev, uint8_t* hard_out,
+                              int32_t* trials_out, int batch, void* stream);
+
 /* Replaces bch_check_cuda::check_async (bch_check_kernel.cu:26-36):
  * hard = B x n_bch/8 bytes, valid_out = B ints (1 = c(x) mod g(x) == 0). */
 int cusat_bch_check(cusat_fec_ctx* ctx, const uint8_t* hard, int32_t* valid_out, int batch,
