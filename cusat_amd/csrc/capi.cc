@@ -244,6 +244,10 @@ int prepare(cusat_fec_ctx* c, int framesize, int rate, int modulation, unsigned 
     }
     xfec_lengths(modulation, c->geo.N, c->con.bits_per_sym, &c->xfec, &c->dlen);
     c->dmax = llr_dmax(modulation, c->con);
+    float rmax = 0.0f;
+    for (int i = 0; i < c->con.n_points; i++)
+        rmax = std::max(rmax, std::sqrt(c->con.xy[2 * i] * c->con.xy[2 * i] + c->con.xy[2 * i + 1] * c->con.xy[2 * i + 1]));
+    c->grid_scale = demap_grid_scale(rmax);
     if (const char* e = getenv("CUSAT_DEMAP_LLR_TARGET")) c->llr_target = (float)atof(e);
     return CUSAT_OK;
 }
@@ -384,6 +388,7 @@ extern "C" int cusat_fec_demap(cusat_fec_ctx* c, const float* sym, int16_t* llr,
     a.N = c->geo.N;
     a.n_mod = c->con.bits_per_sym;
     a.n_points = c->con.n_points;
+    a.grid_scale = c->grid_scale;
     for (int b = 0; b < 8; b++) a.col[b] = c->il.col[b];
     demap_consts(c, noise_var, llr_gain, &a.qpsk_scale, &a.psk8_scale, &a.gain);
     HIP_TRY(launch_demap(a, batch, (hipStream_t)stream));

@@ -25,6 +25,7 @@ struct cusat_fec_ctx {
     cusat::interleaver il;
     int xfec = 0, dlen = 0;
     float dmax = 0, llr_target = 320.0f;
+    float grid_scale = 0;  // symbol grid of the sorted APSK demap (demap_grid_scale)
     cusat::bch_code bch;
     int bch_chunk = 0;
     int device = 0;

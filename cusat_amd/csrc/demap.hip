@@ -148,26 +148,239 @@ __global__ void __launch_bounds__(256) demap_kernel(demap_args a)
     demap_symbol(a, pts, j, reinterpret_cast<const float2*>(a.sym) + (size_t)cw * a.xfec, a.llr + (size_t)cw * a.N);
 }
 
-// Mixed-MODCOD batch: a fixed grid walks the batch's 256-symbol blocks
-// (frame f owns blocks recs[f].blk0 .. recs[f + 1].blk0 - 1; hdr[0] =
-// frames, hdr[1] = blocks, both read from device memory so a captured launch
-// serves any batch).  Every symbol gets exactly the arithmetic of demap_kernel.
+// ---- sorted APSK demap (constellations of 64 points and more) ----
+//
+// Per symbol the arithmetic is demap_symbol's APSK branch, bit for bit.
+// Its second pass adds exp(-(d_i - min_b) g/4) over the points i whose
+// argument stays below 16 for bit b; a point farther than every bit's
+// minimum + 20 / (g/4) adds nothing to any bit.  A wave whose 64 symbols
+// lie close together therefore skips most points: the workgroup sorts the
+// frame's symbols by a 32 x 32 grid cell (Morton order, counting sort in
+// the LDS) and each wave takes 64 consecutive sorted symbols.  Which lane
+// demaps which symbol changes only the speed; results equal demap_kernel's.
+
+#define CUSAT_AS_CONST __attribute__((address_space(4)))
+
+constexpr int SORT_THREADS = 256;
+constexpr int GRID_CELLS = 1024;  // 32 x 32
+
+__device__ __forceinline__ unsigned spread5(unsigned v)
+{
+    v &= 31u;
+    v = (v | (v << 4)) & 0x10Fu;
+    v = (v | (v << 2)) & 0x133u;
+    v = (v | (v << 1)) & 0x155u;
+    return v;
+}
+
+// grid cell of a symbol (Morton order of the 32 x 32 cells); NaN and
+// symbols outside the grid land in the border cells
+__device__ __forceinline__ unsigned grid_cell(float2 y, float scale)
+{
+    const float qx = fminf(fmaxf(fmaf(y.x, scale, 16.0f), 0.0f), 31.0f);
+    const float qy = fminf(fmaxf(fmaf(y.y, scale, 16.0f), 0.0f), 31.0f);
+    return spread5((unsigned)qx) | (spread5((unsigned)qy) << 1);
+}
+
+template <int NM>
+__device__ __forceinline__ void demap_apsk_symbol(CUSAT_AS_CONST const float* pts, float re, float im, float gain,
+                                                  const uint32_t* col, int N, int j, int16_t* __restrict__ llr)
+{
+    constexpr int NP = 1 << NM;
+    float mn0[NM], mn1[NM], s0[NM], s1[NM];
+#pragma unroll
+    for (int b = 0; b < NM; b++) {
+        mn0[b] = 1e30f;
+        mn1[b] = 1e30f;
+        s0[b] = 0.0f;
+        s1[b] = 0.0f;
+    }
+#pragma unroll 8
+    for (int i = 0; i < NP; i++) {
+        const float dr = re - pts[2 * i], di = im - pts[2 * i + 1];
+        const float d = fmaf(dr, dr, di * di);
+#pragma unroll
+        for (int b = 0; b < NM; b++) {
+            if (i & (1 << (NM - 1 - b))) mn1[b] = d < mn1[b] ? d : mn1[b];
+            else                         mn0[b] = d < mn0[b] ? d : mn0[b];
+        }
+    }
+    const float quarter = gain * 0.25f, half = gain * 0.5f;
+    float mx = 0.0f;
+#pragma unroll
+    for (int b = 0; b < NM; b++) mx = fmaxf(mx, fmaxf(mn0[b], mn1[b]));
+    // d >= thr: (d - min_b) g/4 >= 20 (1 - a few ulps) > 16 for every bit b
+    const float thr = mx + 20.0f / quarter;
+    for (int i0 = 0; i0 < NP; i0 += 8) {
+        // eight points per group: one scalar load, then the eight
+        float px[8], py[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            px[k] = pts[2 * (i0 + k)];
+            py[k] = pts[2 * (i0 + k) + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+        const int i = i0 + k;
+        const float dr = re - px[k], di = im - py[k];
+        const float d = fmaf(dr, dr, di * di);
+        if (d < thr) {
+            // branch-free per bit: a term left out is added as +0.0f, an
+            // identity here (the sums start at +0 and every term is > 0)
+#pragma unroll
+            for (int b = 0; b < NM; b++) {
+                const bool pol = (i >> (NM - 1 - b)) & 1;
+                const float dl = (d - (pol ? mn1[b] : mn0[b])) * quarter;
+                const float e = dl < 16.0f ? exp_neg(-dl) : 0.0f;
+                s1[b] = pol ? s1[b] + e : s1[b];
+                s0[b] = pol ? s0[b] : s0[b] + e;
+            }
+        }
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < NM; b++) {
+        const float corr = log_pos(fmaxf(s0[b], 1e-30f)) - log_pos(fmaxf(s1[b], 1e-30f));
+        const float v = fmaf(2.0f, corr, (mn1[b] - mn0[b]) * half);
+        const uint32_t pos = col[b] + (uint32_t)j;
+        if (pos < (uint32_t)N) llr[pos] = to_llr(v);
+    }
+}
+
+// LDS of the sorted path: cell counts / bases, then the sorted symbol indices
+struct sort_lds {
+    int cell[GRID_CELLS];
+    uint16_t order[SORTED_DEMAP_MAX_SYMBOLS];
+    int wave_sum[SORT_THREADS / 64];
+};
+
+// Whole workgroup: one frame of n symbols y (LLRs to llr), constellation pts
+// in global memory, NM bits per symbol.
+template <int NM>
+__device__ __forceinline__ void demap_frame_sorted(sort_lds& L, const float* pts_g, const float2* __restrict__ y,
+                                                   int n, int N, float gain, float scale, const uint32_t* col,
+                                                   int16_t* __restrict__ llr)
+{
+    const int t = threadIdx.x;
+    for (int k = t; k < GRID_CELLS; k += SORT_THREADS) L.cell[k] = 0;
+    __syncthreads();
+    for (int j = t; j < n; j += SORT_THREADS) atomicAdd(&L.cell[grid_cell(y[j], scale)], 1);
+    __syncthreads();
+    // exclusive scan of the 1024 counts: 4 per thread, then over the threads
+    constexpr int PER = GRID_CELLS / SORT_THREADS;
+    int v[PER], run = 0;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        v[k] = L.cell[PER * t + k];
+        run += v[k];
+    }
+    const int lane = t & 63, w = t >> 6;
+    int inc = run;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int o = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += o;
+    }
+    if (lane == 63) L.wave_sum[w] = inc;
+    __syncthreads();
+    int base = inc - run;
+    for (int k = 0; k < w; k++) base += L.wave_sum[k];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        L.cell[PER * t + k] = base;
+        base += v[k];
+    }
+    __syncthreads();
+    for (int j = t; j < n; j += SORT_THREADS) {
+        const int p = atomicAdd(&L.cell[grid_cell(y[j], scale)], 1);
+        L.order[p] = (uint16_t)j;
+    }
+    __syncthreads();
+    const auto* pts = (CUSAT_AS_CONST const float*)pts_g;
+    for (int p = t; p < n; p += SORT_THREADS) {
+        const int j = L.order[p];
+        const float2 s = y[j];
+        demap_apsk_symbol<NM>(pts, s.x, s.y, gain, col, N, j, llr);
+    }
+    __syncthreads();  // the LDS is reused by the next frame
+}
+
+template <int NM>
+__device__ __forceinline__ void demap_frame_sorted_nm(sort_lds& L, const float* pts, const float2* y, int n, int N,
+                                                      float gain, float scale, const uint32_t* col, int16_t* llr)
+{
+    demap_frame_sorted<NM>(L, pts, y, n, N, gain, scale, col, llr);
+}
+
+__device__ __forceinline__ void demap_frame_sorted_any(sort_lds& L, int n_mod, const float* pts, const float2* y,
+                                                       int n, int N, float gain, float scale, const uint32_t* col,
+                                                       int16_t* llr)
+{
+    switch (n_mod) {
+    case 6: demap_frame_sorted_nm<6>(L, pts, y, n, N, gain, scale, col, llr); break;
+    case 7: demap_frame_sorted_nm<7>(L, pts, y, n, N, gain, scale, col, llr); break;
+    case 8: demap_frame_sorted_nm<8>(L, pts, y, n, N, gain, scale, col, llr); break;
+    default: break;
+    }
+}
+
+// cusat_fec_demap of a constellation of the sorted path: one workgroup per frame
+__global__ void __launch_bounds__(SORT_THREADS) demap_sorted_kernel(demap_args a)
+{
+    __shared__ sort_lds L;
+    const int f = blockIdx.x;
+    demap_frame_sorted_any(L, a.n_mod, a.pts, reinterpret_cast<const float2*>(a.sym) + (size_t)f * a.xfec, a.dlen,
+                           a.N, a.gain, a.grid_scale, a.col, a.llr + (size_t)f * a.N);
+}
+
+// Mixed-MODCOD batch, frames of the sorted path: a fixed grid takes them in
+// dlist order (hdr[3] of them, read from device memory so a captured launch
+// serves any batch).
+__global__ void __launch_bounds__(SORT_THREADS)
+    demap_mixed_sorted_kernel(const rx_code* __restrict__ codes, const rx_rec* __restrict__ recs,
+                              const int2* __restrict__ dlist, const rx_scale* __restrict__ snap,
+                              const int* __restrict__ hdr, const float* __restrict__ sym, int16_t* __restrict__ llr)
+{
+    __shared__ sort_lds L;
+    const int nh = hdr[3];
+    for (int k = blockIdx.x; k < nh; k += gridDim.x) {
+        const int f = dlist[k].x;
+        const rx_rec& r = recs[f];
+        const rx_code& c = codes[r.code];
+        const float gain = (r.flags & RX_SCALE_STATE) ? snap[r.code].gain : r.gain;
+        uint32_t col[8];
+#pragma unroll
+        for (int b = 0; b < 8; b++) col[b] = c.col[b];
+        demap_frame_sorted_any(L, c.n_mod, c.pts, reinterpret_cast<const float2*>(sym) + r.sym_off, c.dlen, c.N, gain,
+                               c.grid_scale, col, llr + r.llr_off);
+    }
+}
+
+// Mixed-MODCOD batch, the other frames: a fixed grid walks their 256-symbol
+// blocks in dlist order (frame dlist[k].x owns blocks dlist[k].y ..
+// dlist[k + 1].y - 1; hdr[1] blocks in all, all read from device memory).
+// The frames are in descending cost, so one workgroup's consecutive blocks
+// are mostly of one code and the constellation is reloaded rarely.  Every
+// symbol gets exactly the arithmetic of demap_kernel.
 __global__ void __launch_bounds__(256)
     demap_mixed_kernel(const rx_code* __restrict__ codes, const rx_rec* __restrict__ recs,
-                       const rx_scale* __restrict__ snap, const int* __restrict__ hdr,
-                       const float* __restrict__ sym, int16_t* __restrict__ llr)
+                       const int2* __restrict__ dlist, const rx_scale* __restrict__ snap,
+                       const int* __restrict__ hdr, const float* __restrict__ sym, int16_t* __restrict__ llr)
 {
     __shared__ float pts[512];
-    const int n = hdr[0], total = hdr[1];
-    int cur = -1;  // code whose constellation the LDS holds
+    const int n = hdr[0], total = hdr[1], nh = hdr[3];
+    int cur = -1;    // code whose constellation the LDS holds
+    int k_lo = nh;   // blocks are taken in ascending order: the frame only moves forward
     for (int item = blockIdx.x; item < total; item += gridDim.x) {
-        int lo = 0, hi = n - 1;  // last frame with blk0 <= item
+        int lo = k_lo, hi = n - 1;  // last entry with first block <= item
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if (recs[mid].blk0 <= item) lo = mid;
+            if (dlist[mid].y <= item) lo = mid;
             else hi = mid - 1;
         }
-        const rx_rec& r = recs[lo];
+        k_lo = lo;
+        const int2 e = dlist[lo];
+        const rx_rec& r = recs[e.x];
         const rx_code& c = codes[r.code];
         demap_args a;
         a.pts = c.pts;
@@ -194,7 +407,7 @@ __global__ void __launch_bounds__(256)
             __syncthreads();
             cur = r.code;
         }
-        const int j = (item - r.blk0) * 256 + threadIdx.x;
+        const int j = (item - e.y) * 256 + threadIdx.x;
         if (j < a.dlen)
             demap_symbol(a, pts, j, reinterpret_cast<const float2*>(sym) + r.sym_off, llr + r.llr_off);
     }
@@ -363,15 +576,24 @@ __global__ void __launch_bounds__(64)
 
 hipError_t launch_demap(const demap_args& a, int batch, hipStream_t stream)
 {
+    if (demap_sorted(a.pts != nullptr, a.n_points, a.dlen)) {
+        hipLaunchKernelGGL(demap_sorted_kernel, dim3(batch), dim3(SORT_THREADS), 0, stream, a);
+        return hipGetLastError();
+    }
     dim3 grid((a.dlen + 255) / 256, batch);
     hipLaunchKernelGGL(demap_kernel, grid, dim3(256), 0, stream, a);
     return hipGetLastError();
 }
 
-hipError_t launch_demap_mixed(const rx_code* codes, const rx_rec* recs, const rx_scale* snap, const int* hdr,
-                              const float* sym, int16_t* llr, int grid, hipStream_t stream)
+hipError_t launch_demap_mixed(const rx_code* codes, const rx_rec* recs, const int2* dlist, const rx_scale* snap,
+                              const int* hdr, const float* sym, int16_t* llr, int n_cu, hipStream_t stream)
 {
-    hipLaunchKernelGGL(demap_mixed_kernel, dim3(grid), dim3(256), 0, stream, codes, recs, snap, hdr, sym, llr);
+    hipLaunchKernelGGL(demap_mixed_sorted_kernel, dim3(n_cu * 4), dim3(SORT_THREADS), 0, stream, codes, recs, dlist,
+                       snap, hdr, sym, llr);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(demap_mixed_kernel, dim3(n_cu * 8), dim3(256), 0, stream, codes, recs, dlist, snap, hdr, sym,
+                       llr);
     return hipGetLastError();
 }
 

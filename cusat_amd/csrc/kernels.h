@@ -67,8 +67,23 @@ struct demap_args {
     int xfec, dlen, N, n_mod, n_points;
     uint32_t col[8];
     float qpsk_scale, psk8_scale, gain;
+    float grid_scale;          // sorted APSK demap: cells per unit of the 32 x 32 symbol grid / 2
 };
 hipError_t launch_demap(const demap_args& a, int batch, hipStream_t stream);
+// Constellations demapped by the sorted path (one workgroup per frame, its
+// symbols counting-sorted over a 32 x 32 grid so that each wave holds
+// neighbouring symbols and skips the points none of them needs): APSK
+// tables of at least SORTED_DEMAP_MIN_POINTS points and at most
+// SORTED_DEMAP_MAX_SYMBOLS demapped symbols.
+constexpr int SORTED_DEMAP_MIN_POINTS = 64;
+constexpr int SORTED_DEMAP_MAX_SYMBOLS = 16200;
+inline bool demap_sorted(bool table, int n_points, int dlen)
+{
+    return table && n_points >= SORTED_DEMAP_MIN_POINTS && dlen <= SORTED_DEMAP_MAX_SYMBOLS;
+}
+// grid scale of a constellation of maximum radius rmax: the grid spans
+// [-1.05 rmax, 1.05 rmax] in both axes
+inline float demap_grid_scale(float rmax) { return rmax > 0.0f ? 16.0f / (1.05f * rmax) : 0.0f; }
 
 // Mixed-MODCOD receive batch (cusat_stream_rx): per code and per frame
 // records in device memory.
@@ -80,7 +95,7 @@ struct rx_code {
     int kbch_bytes, hard_bytes;  // k_bch / 8, k_ldpc / 8
     int has_cap;               // the demapper gain is capped at gain_cap
     float gain_cap;
-    int pad;
+    float grid_scale;          // demap_grid_scale of the constellation (sorted path)
     uint32_t col[8];
 };
 struct rx_rec {
@@ -105,9 +120,14 @@ struct rx_event {
 struct rx_scale {
     float qpsk_scale, psk8_scale, gain, snr_lin;
 };
-// hdr[0] = frames, hdr[1] = demap blocks, hdr[2] = SNR events of the batch (device memory)
-hipError_t launch_demap_mixed(const rx_code* codes, const rx_rec* recs, const rx_scale* snap, const int* hdr,
-                              const float* sym, int16_t* llr, int grid, hipStream_t stream);
+// hdr[0] = frames, hdr[1] = demap blocks of the block-walk frames, hdr[2] =
+// SNR events, hdr[3] = frames of the sorted path (device memory).  dlist:
+// per frame (frame number, first block): the sorted-path frames first,
+// costliest first, then the block-walk frames, costliest first (blocks
+// numbered in that order).  Two launches: the sorted-path frames, one
+// workgroup per frame, then the 256-symbol blocks of the others.
+hipError_t launch_demap_mixed(const rx_code* codes, const rx_rec* recs, const int2* dlist, const rx_scale* snap,
+                              const int* hdr, const float* sym, int16_t* llr, int n_cu, hipStream_t stream);
 // est[e] = decision-directed SNR of the frame of every estimate event
 hipError_t launch_dd_estimate(const rx_code* codes, const rx_rec* recs, const rx_event* ev, const int* hdr,
                               const float* sym, float* est, int grid, hipStream_t stream);

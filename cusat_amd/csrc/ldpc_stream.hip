@@ -421,7 +421,7 @@ struct cusat_stream_ctx {
     // plan region, LLR and hard-decision work buffers (all allocated by
     // cusat_stream_create)
     rx_code* d_rx_codes = nullptr;
-    size_t off_rx = 0, off_ev = 0;  // rx header + records; SNR events
+    size_t off_rx = 0, off_dl = 0, off_ev = 0;  // rx header + records; demap order; SNR events
     int max_events = 0;
     rx_scale* d_snap = nullptr;     // per-code demapper scales of the current batch
     float* d_est = nullptr;         // per-event decision-directed estimates
@@ -432,6 +432,7 @@ struct cusat_stream_ctx {
     // plan
     int n_frames = 0, n_jobs = 0;
     std::vector<cusat_stream_frame> frames;
+    std::vector<int> demap_order;  // host scratch of the rx plan
     cusat_stream_tuning tuning{};
 };
 
@@ -567,6 +568,7 @@ int upload_codes(cusat_stream_ctx* sc, bool rx_codes)
         o.pts = formula ? nullptr : c->d_pts;
         o.prbs = c->d_prbs;
         o.dd_pts = c->d_pts;
+        o.grid_scale = c->grid_scale;
         o.has_cap = c->llr_target > 0.0f && c->dmax > 0.0f;
         o.gain_cap = o.has_cap ? 2.0f * c->llr_target / c->dmax : 0.0f;  // as cusat_fec_demap
         o.xfec = c->xfec;
@@ -717,7 +719,8 @@ extern "C" int cusat_stream_create(cusat_stream_ctx** out, cusat_fec_ctx* const*
     sc->off_order = sc->off_jobs + al(sizeof(int2) * max_frames);
     sc->off_rx = sc->off_order + al(sizeof(int) * max_frames);
     sc->max_events = 2 * max_frames + n_codes;
-    sc->off_ev = sc->off_rx + 256 + al(sizeof(rx_rec) * max_frames);
+    sc->off_dl = sc->off_rx + 256 + al(sizeof(rx_rec) * max_frames);
+    sc->off_ev = sc->off_dl + al(sizeof(int2) * max_frames);
     sc->plan_bytes = sc->off_ev + al(sizeof(rx_event) * sc->max_events);
     const size_t bytes = sc->plan_bytes + al(sizeof(code_desc) * n_codes) + al(sizeof(rx_code) * n_codes);
     sc->stage_bytes = sc->plan_bytes + sizeof(code_desc) * n_codes;
@@ -932,17 +935,51 @@ extern "C" int cusat_stream_rx_plan_async(cusat_stream_ctx* sc, const cusat_stre
     if (rc != CUSAT_OK) return rc;
     int* hdr = reinterpret_cast<int*>(img + sc->off_rx);
     rx_rec* rec = reinterpret_cast<rx_rec*>(img + sc->off_rx + 256);
+    // demap order: the frames of the sorted path, then the others, each part
+    // costliest first (arrival order among equals); blocks of the others
+    // numbered in that order
+    int2* dl = reinterpret_cast<int2*>(img + sc->off_dl);
+    int n_sorted = 0, blk = 0;
+    {
+        std::vector<int>& ord = sc->demap_order;
+        ord.resize(n_frames);
+        std::iota(ord.begin(), ord.end(), 0);
+        auto sorted_path = [&](int f) {
+            const cusat_fec_ctx* c = sc->codes[frames[f].code];
+            const bool table = !(c->modulation == CUSAT_MOD_QPSK || c->modulation == CUSAT_MOD_8PSK);
+            return demap_sorted(table, c->con.n_points, c->dlen);
+        };
+        auto cost = [&](int f) {
+            const cusat_fec_ctx* c = sc->codes[frames[f].code];
+            const bool table = !(c->modulation == CUSAT_MOD_QPSK || c->modulation == CUSAT_MOD_8PSK);
+            return (int64_t)c->dlen * (table ? c->con.n_points * c->con.bits_per_sym : 1);
+        };
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+            const bool sa = sorted_path(a), sb = sorted_path(b);
+            if (sa != sb) return sa;
+            return cost(a) > cost(b);
+        });
+        for (int k = 0; k < n_frames; k++) {
+            const int f = ord[k];
+            if (sorted_path(f)) {
+                dl[k] = make_int2(f, 0);
+                n_sorted++;
+            } else {
+                dl[k] = make_int2(f, blk);
+                blk += (sc->codes[frames[f].code]->dlen + 255) / 256;
+            }
+        }
+    }
     // the decoder's frames: LLRs and hard decisions packed in the work buffers
     sc->frames.resize(n_frames);
     int64_t lo = 0, ho = 0;
-    int blk = 0;
     for (int f = 0; f < n_frames; f++) {
         const cusat_fec_ctx* c = sc->codes[frames[f].code];
         sc->frames[f] = cusat_stream_frame{frames[f].code, 0, lo, ho};
         rx_rec& r = rec[f];
         memset(&r, 0, sizeof r);
         r.code = frames[f].code;
-        r.blk0 = blk;
+        r.blk0 = -1;
         r.sym_off = frames[f].sym_offset;
         r.llr_off = lo;
         r.hard_off = ho;
@@ -954,12 +991,14 @@ extern "C" int cusat_stream_rx_plan_async(cusat_stream_ctx* sc, const cusat_stre
                                    &r.gain);
         lo += c->geo.N;
         ho += c->geo.K / 8;
-        blk += (c->dlen + 255) / 256;
     }
+    for (int k = 0; k < n_frames; k++)
+        if (dl[k].y >= 0 && k >= n_sorted) rec[dl[k].x].blk0 = dl[k].y;
     memset(hdr, 0, 256);
     hdr[0] = n_frames;
     hdr[1] = blk;
     hdr[2] = n_events;
+    hdr[3] = n_sorted;
     rx_event* ev = reinterpret_cast<rx_event*>(img + sc->off_ev);
     for (int e = 0; e < n_events; e++)
         ev[e] = rx_event{events[e].code, events[e].frame, events[e].snr_lin, events[e].snr_db};
@@ -975,6 +1014,8 @@ extern "C" int cusat_stream_rx_plan_async(cusat_stream_ctx* sc, const cusat_stre
         HIP_TRY(copy_plan(sc, img, n_frames, nj, st, true));
         HIP_TRY(hipMemcpyAsync(d + sc->off_rx, img + sc->off_rx, 256 + n_frames * sizeof(rx_rec),
                                hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(d + sc->off_dl, img + sc->off_dl, n_frames * sizeof(int2), hipMemcpyHostToDevice,
+                               st));
         if (n_events)
             HIP_TRY(hipMemcpyAsync(d + sc->off_ev, img + sc->off_ev, n_events * sizeof(rx_event),
                                    hipMemcpyHostToDevice, st));
@@ -1011,7 +1052,9 @@ extern "C" int cusat_stream_rx_demap(cusat_stream_ctx* sc, const float* sym, voi
     if (rc != CUSAT_OK) return rc;
     const int* hdr = rx_header(sc);
     const rx_rec* rec = reinterpret_cast<const rx_rec*>(hdr + 64);
-    HIP_TRY(launch_demap_mixed(sc->d_rx_codes, rec, sc->d_snap, hdr, sym, sc->d_llr_work, sc->n_cu * 4,
+    const int2* dlist =
+        reinterpret_cast<const int2*>(reinterpret_cast<const unsigned char*>(sc->d_counters) + sc->off_dl);
+    HIP_TRY(launch_demap_mixed(sc->d_rx_codes, rec, dlist, sc->d_snap, hdr, sym, sc->d_llr_work, sc->n_cu,
                                (hipStream_t)stream));
     return CUSAT_OK;
 }
