@@ -64,10 +64,108 @@ __device__ __forceinline__ int16_t to_llr(float v)
     return (int16_t)v;
 }
 
+#define CUSAT_AS_CONST __attribute__((address_space(4)))
+
+// LLRs of one symbol of an APSK table of 2^NM points (pts: the
+// constellation, read through scalar loads, eight points per load):
+// two-pass log-MAP.  Pass 1: per bit b the smallest squared distance to a
+// point with bit b = 0 (mn0) and = 1 (mn1).  Pass 2, in point order: for
+// each bit the sum over the points of that bit value of exp(-dl),
+// dl = (d - min) g/4, over the points with dl < 16.  LLR = (mn1 - mn0) g/2
+// + 2 (log sum0 - log sum1).
+//
+// Pass 2 skips a point, for the whole wave, when no lane is near it: for
+// the eight points of a group the top NM - 3 label bits are fixed, so a
+// lane's bound is the largest minimum over those bits' values and both
+// values of the three low bits, + 20 / (g/4); a point beyond it has
+// dl >= 20 (1 - a few ulps) > 16 for every bit and adds nothing.  Inside,
+// the per-bit update is branch-free: a term left out adds +0.0f, an
+// identity here (the sums start at +0 and every term is > 0).  The values
+// therefore equal the plain two-pass loop over every point.
+template <int NM>
+__device__ __forceinline__ void demap_apsk_symbol(CUSAT_AS_CONST const float* pts, float re, float im, float gain,
+                                                  const uint32_t* col, int N, int j, int16_t* __restrict__ llr)
+{
+    constexpr int NP = 1 << NM;
+    constexpr int LOW = NM < 3 ? NM : 3;  // label bits that vary within a group of eight
+    float mn0[NM], mn1[NM], s0[NM], s1[NM];
+#pragma unroll
+    for (int b = 0; b < NM; b++) {
+        mn0[b] = 1e30f;
+        mn1[b] = 1e30f;
+        s0[b] = 0.0f;
+        s1[b] = 0.0f;
+    }
+#pragma unroll 8
+    for (int i = 0; i < NP; i++) {
+        const float dr = re - pts[2 * i], di = im - pts[2 * i + 1];
+        const float d = fmaf(dr, dr, di * di);
+#pragma unroll
+        for (int b = 0; b < NM; b++) {
+            if (i & (1 << (NM - 1 - b))) mn1[b] = d < mn1[b] ? d : mn1[b];
+            else                         mn0[b] = d < mn0[b] ? d : mn0[b];
+        }
+    }
+    const float quarter = gain * 0.25f, half = gain * 0.5f;
+    const float margin = 20.0f / quarter;
+    float low_max = 0.0f;
+#pragma unroll
+    for (int b = NM - LOW; b < NM; b++) low_max = fmaxf(low_max, fmaxf(mn0[b], mn1[b]));
+    for (int i0 = 0; i0 < NP; i0 += 8) {
+        float tg = low_max;
+#pragma unroll
+        for (int b = 0; b < NM - LOW; b++) tg = fmaxf(tg, ((i0 >> (NM - 1 - b)) & 1) ? mn1[b] : mn0[b]);
+        tg += margin;
+        float px[8], py[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            px[k] = pts[2 * (i0 + k)];
+            py[k] = pts[2 * (i0 + k) + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < (NP < 8 ? NP : 8); k++) {
+            const int i = i0 + k;
+            const float dr = re - px[k], di = im - py[k];
+            const float d = fmaf(dr, dr, di * di);
+            if (d < tg) {
+#pragma unroll
+                for (int b = 0; b < NM; b++) {
+                    const bool pol = (i >> (NM - 1 - b)) & 1;
+                    const float dl = (d - (pol ? mn1[b] : mn0[b])) * quarter;
+                    const float e = dl < 16.0f ? exp_neg(-dl) : 0.0f;
+                    s1[b] = pol ? s1[b] + e : s1[b];
+                    s0[b] = pol ? s0[b] : s0[b] + e;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < NM; b++) {
+        const float corr = log_pos(fmaxf(s0[b], 1e-30f)) - log_pos(fmaxf(s1[b], 1e-30f));
+        const float v = fmaf(2.0f, corr, (mn1[b] - mn0[b]) * half);
+        const uint32_t pos = col[b] + (uint32_t)j;
+        if (pos < (uint32_t)N) llr[pos] = to_llr(v);
+    }
+}
+
+__device__ __forceinline__ void demap_apsk_any(int n_mod, const float* pts_g, float re, float im, float gain,
+                                               const uint32_t* col, int N, int j, int16_t* llr)
+{
+    const auto* pts = (CUSAT_AS_CONST const float*)pts_g;
+    switch (n_mod) {
+    case 3: demap_apsk_symbol<3>(pts, re, im, gain, col, N, j, llr); break;
+    case 4: demap_apsk_symbol<4>(pts, re, im, gain, col, N, j, llr); break;
+    case 5: demap_apsk_symbol<5>(pts, re, im, gain, col, N, j, llr); break;
+    case 6: demap_apsk_symbol<6>(pts, re, im, gain, col, N, j, llr); break;
+    case 7: demap_apsk_symbol<7>(pts, re, im, gain, col, N, j, llr); break;
+    case 8: demap_apsk_symbol<8>(pts, re, im, gain, col, N, j, llr); break;
+    default: break;
+    }
+}
+
 // LLRs of symbol j of one frame (sym = the frame's symbols, llr = its N
-// LLRs); pts = the constellation in the LDS (APSK).
-__device__ __forceinline__ void demap_symbol(const demap_args& a, const float* pts, int j, const float2* sym,
-                                             int16_t* llr)
+// LLRs): the QPSK and 8PSK closed forms, else the APSK table a.pts.
+__device__ __forceinline__ void demap_symbol(const demap_args& a, int j, const float2* sym, int16_t* llr)
 {
     const float2 y = sym[j];
     const float re = y.x, im = y.y;
@@ -86,82 +184,28 @@ __device__ __forceinline__ void demap_symbol(const demap_args& a, const float* p
         llr[a.col[1] + j] = to_llr(cr * a.psk8_scale);
         llr[a.col[2] + j] = to_llr(ci * a.psk8_scale);
     } else {
-        const int nm = a.n_mod;
-        float mn0[8], mn1[8], s0[8], s1[8];
-#pragma unroll
-        for (int b = 0; b < 8; b++) {
-            mn0[b] = 1e30f;
-            mn1[b] = 1e30f;
-            s0[b] = 0.0f;
-            s1[b] = 0.0f;
-        }
-        for (int i = 0; i < a.n_points; i++) {
-            const float dr = re - pts[2 * i], di = im - pts[2 * i + 1];
-            const float d = fmaf(dr, dr, di * di);
-#pragma unroll
-            for (int b = 0; b < 8; b++) {
-                if (b < nm) {
-                    if (i & (1 << (nm - 1 - b))) mn1[b] = d < mn1[b] ? d : mn1[b];
-                    else                         mn0[b] = d < mn0[b] ? d : mn0[b];
-                }
-            }
-        }
-        const float quarter = a.gain * 0.25f, half = a.gain * 0.5f;
-        for (int i = 0; i < a.n_points; i++) {
-            const float dr = re - pts[2 * i], di = im - pts[2 * i + 1];
-            const float d = fmaf(dr, dr, di * di);
-#pragma unroll
-            for (int b = 0; b < 8; b++) {
-                if (b < nm) {
-                    if (i & (1 << (nm - 1 - b))) {
-                        const float dl = (d - mn1[b]) * quarter;
-                        if (dl < 16.0f) s1[b] += exp_neg(-dl);
-                    } else {
-                        const float dl = (d - mn0[b]) * quarter;
-                        if (dl < 16.0f) s0[b] += exp_neg(-dl);
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int b = 0; b < 8; b++) {
-            if (b < nm) {
-                const float corr = log_pos(fmaxf(s0[b], 1e-30f)) - log_pos(fmaxf(s1[b], 1e-30f));
-                const float v = fmaf(2.0f, corr, (mn1[b] - mn0[b]) * half);
-                const uint32_t pos = a.col[b] + (uint32_t)j;
-                if (pos < (uint32_t)a.N) llr[pos] = to_llr(v);
-            }
-        }
+        demap_apsk_any(a.n_mod, a.pts, re, im, a.gain, a.col, a.N, j, llr);
     }
 }
 
 __global__ void __launch_bounds__(256) demap_kernel(demap_args a)
 {
-    __shared__ float pts[512];
-    if (a.pts)
-        for (int i = threadIdx.x; i < 2 * a.n_points; i += blockDim.x) pts[i] = a.pts[i];
-    __syncthreads();
-
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int cw = blockIdx.y;
     if (j >= a.dlen) return;
-    demap_symbol(a, pts, j, reinterpret_cast<const float2*>(a.sym) + (size_t)cw * a.xfec, a.llr + (size_t)cw * a.N);
+    demap_symbol(a, j, reinterpret_cast<const float2*>(a.sym) + (size_t)cw * a.xfec, a.llr + (size_t)cw * a.N);
 }
 
 // ---- sorted APSK demap (constellations of 64 points and more) ----
 //
-// Per symbol the arithmetic is demap_symbol's APSK branch, bit for bit.
-// Its second pass adds exp(-(d_i - min_b) g/4) over the points i whose
-// argument stays below 16 for bit b; a point farther than every bit's
-// minimum + 20 / (g/4) adds nothing to any bit.  A wave whose 64 symbols
-// lie close together therefore skips most points: the workgroup sorts the
-// frame's symbols by a 32 x 32 grid cell (Morton order, counting sort in
-// the LDS) and each wave takes 64 consecutive sorted symbols.  Which lane
-// demaps which symbol changes only the speed; results equal demap_kernel's.
+// demap_apsk_symbol skips a point for a wave only when no lane is near
+// it, so it pays off when a wave's 64 symbols lie close together: one
+// workgroup per frame sorts the frame's symbols by a 32 x 32 grid cell
+// (Morton order, counting sort in the LDS) and each wave takes 64
+// consecutive sorted symbols.  Which lane demaps which symbol changes only
+// the speed; the values are demap_kernel's.
 
-#define CUSAT_AS_CONST __attribute__((address_space(4)))
-
-constexpr int SORT_THREADS = 256;
+constexpr int SORT_THREADS = 1024;
 constexpr int GRID_CELLS = 1024;  // 32 x 32
 
 __device__ __forceinline__ unsigned spread5(unsigned v)
@@ -182,72 +226,7 @@ __device__ __forceinline__ unsigned grid_cell(float2 y, float scale)
     return spread5((unsigned)qx) | (spread5((unsigned)qy) << 1);
 }
 
-template <int NM>
-__device__ __forceinline__ void demap_apsk_symbol(CUSAT_AS_CONST const float* pts, float re, float im, float gain,
-                                                  const uint32_t* col, int N, int j, int16_t* __restrict__ llr)
-{
-    constexpr int NP = 1 << NM;
-    float mn0[NM], mn1[NM], s0[NM], s1[NM];
-#pragma unroll
-    for (int b = 0; b < NM; b++) {
-        mn0[b] = 1e30f;
-        mn1[b] = 1e30f;
-        s0[b] = 0.0f;
-        s1[b] = 0.0f;
-    }
-#pragma unroll 8
-    for (int i = 0; i < NP; i++) {
-        const float dr = re - pts[2 * i], di = im - pts[2 * i + 1];
-        const float d = fmaf(dr, dr, di * di);
-#pragma unroll
-        for (int b = 0; b < NM; b++) {
-            if (i & (1 << (NM - 1 - b))) mn1[b] = d < mn1[b] ? d : mn1[b];
-            else                         mn0[b] = d < mn0[b] ? d : mn0[b];
-        }
-    }
-    const float quarter = gain * 0.25f, half = gain * 0.5f;
-    float mx = 0.0f;
-#pragma unroll
-    for (int b = 0; b < NM; b++) mx = fmaxf(mx, fmaxf(mn0[b], mn1[b]));
-    // d >= thr: (d - min_b) g/4 >= 20 (1 - a few ulps) > 16 for every bit b
-    const float thr = mx + 20.0f / quarter;
-    for (int i0 = 0; i0 < NP; i0 += 8) {
-        // eight points per group: one scalar load, then the eight
-        float px[8], py[8];
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            px[k] = pts[2 * (i0 + k)];
-            py[k] = pts[2 * (i0 + k) + 1];
-        }
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-        const int i = i0 + k;
-        const float dr = re - px[k], di = im - py[k];
-        const float d = fmaf(dr, dr, di * di);
-        if (d < thr) {
-            // branch-free per bit: a term left out is added as +0.0f, an
-            // identity here (the sums start at +0 and every term is > 0)
-#pragma unroll
-            for (int b = 0; b < NM; b++) {
-                const bool pol = (i >> (NM - 1 - b)) & 1;
-                const float dl = (d - (pol ? mn1[b] : mn0[b])) * quarter;
-                const float e = dl < 16.0f ? exp_neg(-dl) : 0.0f;
-                s1[b] = pol ? s1[b] + e : s1[b];
-                s0[b] = pol ? s0[b] : s0[b] + e;
-            }
-        }
-        }
-    }
-#pragma unroll
-    for (int b = 0; b < NM; b++) {
-        const float corr = log_pos(fmaxf(s0[b], 1e-30f)) - log_pos(fmaxf(s1[b], 1e-30f));
-        const float v = fmaf(2.0f, corr, (mn1[b] - mn0[b]) * half);
-        const uint32_t pos = col[b] + (uint32_t)j;
-        if (pos < (uint32_t)N) llr[pos] = to_llr(v);
-    }
-}
-
-// LDS of the sorted path: cell counts / bases, then the sorted symbol indices
+// LDS of the sorted path: cell counts / bases, the sorted symbol indices
 struct sort_lds {
     int cell[GRID_CELLS];
     uint16_t order[SORTED_DEMAP_MAX_SYMBOLS];
@@ -255,27 +234,21 @@ struct sort_lds {
 };
 
 // Whole workgroup: one frame of n symbols y (LLRs to llr), constellation pts
-// in global memory, NM bits per symbol.
-template <int NM>
-__device__ __forceinline__ void demap_frame_sorted(sort_lds& L, const float* pts_g, const float2* __restrict__ y,
-                                                   int n, int N, float gain, float scale, const uint32_t* col,
-                                                   int16_t* __restrict__ llr)
+// in global memory, n_mod bits per symbol.
+__device__ __forceinline__ void demap_frame_sorted(sort_lds& L, int n_mod, const float* pts,
+                                                   const float2* __restrict__ y, int n, int N, float gain,
+                                                   float scale, const uint32_t* col, int16_t* __restrict__ llr)
 {
+    static_assert(GRID_CELLS == SORT_THREADS, "one grid cell per thread in the scan");
     const int t = threadIdx.x;
-    for (int k = t; k < GRID_CELLS; k += SORT_THREADS) L.cell[k] = 0;
+    L.cell[t] = 0;
     __syncthreads();
     for (int j = t; j < n; j += SORT_THREADS) atomicAdd(&L.cell[grid_cell(y[j], scale)], 1);
     __syncthreads();
-    // exclusive scan of the 1024 counts: 4 per thread, then over the threads
-    constexpr int PER = GRID_CELLS / SORT_THREADS;
-    int v[PER], run = 0;
-#pragma unroll
-    for (int k = 0; k < PER; k++) {
-        v[k] = L.cell[PER * t + k];
-        run += v[k];
-    }
+    // exclusive scan of the counts: within each wave, then over the waves
+    const int v = L.cell[t];
     const int lane = t & 63, w = t >> 6;
-    int inc = run;
+    int inc = v;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
         const int o = __shfl_up(inc, d, 64);
@@ -283,45 +256,21 @@ __device__ __forceinline__ void demap_frame_sorted(sort_lds& L, const float* pts
     }
     if (lane == 63) L.wave_sum[w] = inc;
     __syncthreads();
-    int base = inc - run;
+    int base = inc - v;
     for (int k = 0; k < w; k++) base += L.wave_sum[k];
-#pragma unroll
-    for (int k = 0; k < PER; k++) {
-        L.cell[PER * t + k] = base;
-        base += v[k];
-    }
+    L.cell[t] = base;
     __syncthreads();
     for (int j = t; j < n; j += SORT_THREADS) {
         const int p = atomicAdd(&L.cell[grid_cell(y[j], scale)], 1);
         L.order[p] = (uint16_t)j;
     }
     __syncthreads();
-    const auto* pts = (CUSAT_AS_CONST const float*)pts_g;
     for (int p = t; p < n; p += SORT_THREADS) {
         const int j = L.order[p];
         const float2 s = y[j];
-        demap_apsk_symbol<NM>(pts, s.x, s.y, gain, col, N, j, llr);
+        demap_apsk_any(n_mod, pts, s.x, s.y, gain, col, N, j, llr);
     }
     __syncthreads();  // the LDS is reused by the next frame
-}
-
-template <int NM>
-__device__ __forceinline__ void demap_frame_sorted_nm(sort_lds& L, const float* pts, const float2* y, int n, int N,
-                                                      float gain, float scale, const uint32_t* col, int16_t* llr)
-{
-    demap_frame_sorted<NM>(L, pts, y, n, N, gain, scale, col, llr);
-}
-
-__device__ __forceinline__ void demap_frame_sorted_any(sort_lds& L, int n_mod, const float* pts, const float2* y,
-                                                       int n, int N, float gain, float scale, const uint32_t* col,
-                                                       int16_t* llr)
-{
-    switch (n_mod) {
-    case 6: demap_frame_sorted_nm<6>(L, pts, y, n, N, gain, scale, col, llr); break;
-    case 7: demap_frame_sorted_nm<7>(L, pts, y, n, N, gain, scale, col, llr); break;
-    case 8: demap_frame_sorted_nm<8>(L, pts, y, n, N, gain, scale, col, llr); break;
-    default: break;
-    }
 }
 
 // cusat_fec_demap of a constellation of the sorted path: one workgroup per frame
@@ -329,8 +278,8 @@ __global__ void __launch_bounds__(SORT_THREADS) demap_sorted_kernel(demap_args a
 {
     __shared__ sort_lds L;
     const int f = blockIdx.x;
-    demap_frame_sorted_any(L, a.n_mod, a.pts, reinterpret_cast<const float2*>(a.sym) + (size_t)f * a.xfec, a.dlen,
-                           a.N, a.gain, a.grid_scale, a.col, a.llr + (size_t)f * a.N);
+    demap_frame_sorted(L, a.n_mod, a.pts, reinterpret_cast<const float2*>(a.sym) + (size_t)f * a.xfec, a.dlen, a.N,
+                       a.gain, a.grid_scale, a.col, a.llr + (size_t)f * a.N);
 }
 
 // Mixed-MODCOD batch, frames of the sorted path: a fixed grid takes them in
@@ -351,26 +300,23 @@ __global__ void __launch_bounds__(SORT_THREADS)
         uint32_t col[8];
 #pragma unroll
         for (int b = 0; b < 8; b++) col[b] = c.col[b];
-        demap_frame_sorted_any(L, c.n_mod, c.pts, reinterpret_cast<const float2*>(sym) + r.sym_off, c.dlen, c.N, gain,
-                               c.grid_scale, col, llr + r.llr_off);
+        demap_frame_sorted(L, c.n_mod, c.pts, reinterpret_cast<const float2*>(sym) + r.sym_off, c.dlen, c.N, gain,
+                           c.grid_scale, col, llr + r.llr_off);
     }
 }
 
 // Mixed-MODCOD batch, the other frames: a fixed grid walks their 256-symbol
 // blocks in dlist order (frame dlist[k].x owns blocks dlist[k].y ..
 // dlist[k + 1].y - 1; hdr[1] blocks in all, all read from device memory).
-// The frames are in descending cost, so one workgroup's consecutive blocks
-// are mostly of one code and the constellation is reloaded rarely.  Every
-// symbol gets exactly the arithmetic of demap_kernel.
+// The frames are in descending cost, so the heavy blocks go first and the
+// tail is cheap.  Every symbol gets exactly the arithmetic of demap_kernel.
 __global__ void __launch_bounds__(256)
     demap_mixed_kernel(const rx_code* __restrict__ codes, const rx_rec* __restrict__ recs,
                        const int2* __restrict__ dlist, const rx_scale* __restrict__ snap,
                        const int* __restrict__ hdr, const float* __restrict__ sym, int16_t* __restrict__ llr)
 {
-    __shared__ float pts[512];
     const int n = hdr[0], total = hdr[1], nh = hdr[3];
-    int cur = -1;    // code whose constellation the LDS holds
-    int k_lo = nh;   // blocks are taken in ascending order: the frame only moves forward
+    int k_lo = nh;  // blocks are taken in ascending order: the frame only moves forward
     for (int item = blockIdx.x; item < total; item += gridDim.x) {
         int lo = k_lo, hi = n - 1;  // last entry with first block <= item
         while (lo < hi) {
@@ -401,15 +347,8 @@ __global__ void __launch_bounds__(256)
             a.psk8_scale = r.psk8_scale;
             a.gain = r.gain;
         }
-        if (c.pts && r.code != cur) {
-            __syncthreads();  // every lane is done with the previous table
-            for (int i = threadIdx.x; i < 2 * c.n_points; i += blockDim.x) pts[i] = c.pts[i];
-            __syncthreads();
-            cur = r.code;
-        }
         const int j = (item - e.y) * 256 + threadIdx.x;
-        if (j < a.dlen)
-            demap_symbol(a, pts, j, reinterpret_cast<const float2*>(sym) + r.sym_off, llr + r.llr_off);
+        if (j < a.dlen) demap_symbol(a, j, reinterpret_cast<const float2*>(sym) + r.sym_off, llr + r.llr_off);
     }
 }
 

@@ -147,14 +147,19 @@ class RxStream:
         self.sym = torch.empty((int(xf.sum()), 2), dtype=torch.float32, device="cuda")
         sent = torch.empty(self.out_bytes, dtype=torch.uint8, device="cuda")
         cw_all = torch.empty(int((self.K // 8).sum()), dtype=torch.uint8, device="cuda")
-        for f in range(n):
-            c, j = self.code[f], self.row[f]
-            so, x = int(self.sym_off[f]), int(xf[f])
-            self.sym[so:so + x] = sym_g[c][j]
-            oo, d = int(self.out_off[f]), int(self.dlen[f])
-            sent[oo:oo + d] = pay_g[c][j]
-            ho, kb8 = int(self.hard_off[f]), int(self.K[f] // 8)
-            cw_all[ho:ho + kb8] = cw_g[c][j, :kb8]
+
+        def scatter(dst, src_rows, starts, width):
+            # rows of one group into their arrival-order slots, one indexed copy
+            idx = torch.from_numpy((starts[:, None] + np.arange(width)[None, :]).reshape(-1)).cuda()
+            dst.index_copy_(0, idx, src_rows.reshape(idx.numel(), *dst.shape[1:]))
+
+        for c in range(len(self.keys)):
+            fr = np.nonzero(self.code == c)[0]
+            fr = fr[np.argsort(self.row[fr])]  # group row order
+            scatter(self.sym, sym_g[c], self.sym_off[fr], int(xf[fr[0]]))
+            scatter(sent, pay_g[c], self.out_off[fr], int(self.dlen[fr[0]]))
+            kb8 = int(self.K[fr[0]] // 8)
+            scatter(cw_all, cw_g[c][:, :kb8].contiguous(), self.hard_off[fr], kb8)
         torch.cuda.synchronize()
         self.sent = sent.cpu().numpy()
         self.cw = cw_all.cpu().numpy()  # the BCH codewords (= systematic LDPC info part)
