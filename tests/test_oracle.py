@@ -326,6 +326,176 @@ def test_oracle_vs_independent_restatement(name, fs, rate, es, T, use_sched):
     np.testing.assert_array_equal(h_o[0], h_p)
 
 
+def _np_layered(name, llr, T, use_sched=True):
+    """A second independent restatement, vectorised with numpy so that
+    normal frames run many iterations in seconds.  The same semantics as
+    _py_layered, formulated differently: per layer an edge matrix [360,
+    cnt] (checks q j + i, edges in the append order (r, jj, address index)
+    of ldpc_decode_kernel.cu:1006-1019); the check node as the extrinsic
+    minimum from the two smallest magnitudes of a sort, the sign from the
+    count of negative inputs among the other edges; data writes applied per
+    (lane, circulant) (the 360 VNs of one circulant are distinct), then
+    the parity writes from the read-phase values."""
+    t = TABLES[name]
+    N, K = t["N"], t["K"]
+    R = N - K
+    q = R // 360
+    rows = t["rows"]
+    # every edge: check, vn, (r, jj, n) for the append order, circulant rank
+    layer_circ = {}
+    for r, row in enumerate(rows):
+        for n, a in enumerate(row):
+            layer_circ.setdefault(a % q, []).append((r, n, a))
+    E, RK = {}, {}
+    jj = np.arange(360)
+    for i in range(q):
+        circs = layer_circ.get(i, [])
+        if not circs:
+            E[i] = np.zeros((360, 0), np.int64)
+            RK[i] = np.zeros((360, 0), np.int64)
+            continue
+        # check q j + i receives from circulant (r, n, a) the bit 360 r + jj
+        # with (a + jj q) mod R = q j + i
+        vn = np.zeros((360, len(circs)), np.int64)
+        key = np.zeros((360, len(circs)), np.int64)
+        for k, (r, n, a) in enumerate(circs):
+            j_of_jj = ((a + jj * q) % R - i) // q
+            vn[j_of_jj, k] = 360 * r + jj
+            key[j_of_jj, k] = (r * 360 + jj) * 64 + n
+        order = np.argsort(key, axis=1, kind="stable")
+        E[i] = np.take_along_axis(vn, order, 1)
+        RK[i] = order  # rank of the circulant (layer order) of each edge
+    if use_sched and name in SCHEDS:
+        steps = [tuple(p) for p in SCHEDS[name]["pairs"]]
+    else:
+        steps = [(i, -1) for i in range(q)]
+    jq = np.arange(360) * q
+
+    def sat(v):
+        return np.clip(v, -32768, 32767)
+
+    app_d = np.maximum(llr[:K].astype(np.int64), -32767)
+    app_p = np.maximum(llr[K:].astype(np.int64), -32767)
+    msg = {}
+    res = -1
+    for it in range(T + 1):
+        ok = True
+        for i in range(q):
+            c = jq + i
+            neg = (app_d[E[i]] < 0).sum(1) + (app_p[c] < 0)
+            neg = neg + np.where(c > 0, app_p[np.maximum(c - 1, 0)] < 0, False)
+            if (neg & 1).any():
+                ok = False
+                break
+        if ok:
+            res = T - it
+            break
+        if it == T:
+            break
+        for step in steps:
+            reads = []
+            for i in step:
+                if i < 0:
+                    continue
+                c = jq + i
+                cols = [app_d[E[i]], app_p[c][:, None]]
+                has2 = c > 0
+                cols.append(np.where(has2, app_p[np.maximum(c - 1, 0)], 0)[:, None])
+                vals = np.concatenate(cols, 1)
+                old = msg.get(i, np.zeros_like(vals))
+                x = sat(vals - old)
+                mag = np.maximum(np.minimum(np.abs(x), 32767) - 1, 0)
+                neg = x < 0
+                # check (0, 0) has no second parity edge: out of the minimum
+                # and the sign
+                if i == 0:
+                    mag[0, -1] = 1 << 20
+                    neg[0, -1] = False
+                srt = np.sort(mag, 1)
+                m = np.where(mag == srt[:, :1], srt[:, 1:2], srt[:, :1])
+                sgn = (neg.sum(1, keepdims=True) - neg) & 1
+                y = np.clip(np.where(sgn == 1, -m, m), -127, 127)
+                if i == 0:
+                    y[0, -1] = 0
+                msg[i] = y
+                reads.append((i, c, has2, x, y, old))
+            for i, c, has2, x, y, old in reads:
+                ne = E[i].shape[1]
+                d = y[:, :ne] - old[:, :ne]
+                for k in range(ne):
+                    cols = np.nonzero(RK[i] == k)
+                    v = E[i][cols]
+                    app_d[v] = sat(app_d[v] + d[cols])
+            for i, c, has2, x, y, old in reads:
+                ne = E[i].shape[1]
+                app_p[c] = sat(x[:, ne] + y[:, ne])
+                cc = c[has2] - 1
+                app_p[cc] = sat(x[has2, ne + 1] + y[has2, ne + 1])
+    hard = np.packbits((app_d < 0).astype(np.uint8))
+    return hard, res
+
+
+def test_vectorised_restatement_equals_python_restatement():
+    """The two independent restatements agree (short codes, noisy frames)."""
+    for name, fs, rate, es, T in [("s2_c4", 1, "1/2", 1.0, 6), ("s2_c9", 1, "5/6", 8.5, 4),
+                                  ("s2_b9", 0, "5/6", 6.6, 2)]:
+        code = oc.Code(name)
+        sym, _, _ = ca.gen_frames(fs, rate, "QPSK", 17, 1, es)
+        snr = np.float32(10.0) ** (np.float32(es) / np.float32(10.0))
+        qs = float(np.float32(2 * np.float32(1.4142135623730951)) / (np.float32(1.0) / snr))
+        llr = oc.demap(sym, 2, None, np.array([0, 1, 0, 0, 0, 0, 0, 0], np.uint32), code.N, qs, 0, 0)[0]
+        h_p, r_p = _py_layered(name, llr, T)
+        h_n, r_n = _np_layered(name, llr, T)
+        assert r_p == r_n, name
+        np.testing.assert_array_equal(h_p, h_n)
+
+
+def _rail_frame(fs, rate, N, seed, K=None):
+    """A codeword whose bits sit either at the int16 rails (+32767, -32768
+    and just inside, correct signs) or at small magnitudes with a few of
+    those signs wrong (3 % x (1 - K/N) of the small ones): the decoder
+    corrects them while the APP sums and the parity writes of the railed
+    bits saturate."""
+    _, _, cw = ca.gen_frames(fs, rate, "QPSK", seed, 1, 1000.0, want_cw=True)
+    bits = np.unpackbits(cw[0])[:N]
+    rng = np.random.default_rng(seed)
+    big = rng.random(N) < 0.7
+    mag = np.where(big, rng.choice([32768, 32767, 32700], N), rng.integers(4, 41, N))
+    wrong = ~big & (rng.random(N) < 0.03 * (1.0 - (K or N // 2) / N))
+    s = np.where(bits == 0, 1, -1) * np.where(wrong, -1, 1)
+    return np.clip(s * mag, -32768, 32767).astype(np.int16)
+
+
+@pytest.mark.parametrize("name,fs,rate,th,T", [
+    ("s2_b4", 0, "1/2", 1.80, 12),   # config 1's code (QPSK 1/2 normal)
+    ("s2_b7", 0, "3/4", 4.43, 12),   # config 2's code (3/4 normal)
+    ("s2_b6", 0, "2/3", 3.50, 12),   # config 4's code (2/3 normal, t = 10)
+    ("s2_b11", 0, "9/10", 6.72, 10),  # 9/10: lane-pair collisions
+])
+def test_oracle_vs_vectorised_restatement_normal_frames(name, fs, rate, th, T):
+    """The oracle's hard bits and remaining-trial count equal the vectorised
+    restatement on normal frames of the configs' codes with T >= 10: a
+    QPSK frame at the MODCOD's threshold + 0.5 dB (converges after a few
+    iterations), one at threshold - 0.4 dB (fails after T), and a frame at
+    the int16 rails."""
+    code = oc.Code(name)
+    frames = []
+    for es, seed in ((th + 0.5, 29), (th - 0.4, 30)):
+        sym, _, _ = ca.gen_frames(fs, rate, "QPSK", seed, 1, es)
+        snr = np.float32(10.0) ** (np.float32(es) / np.float32(10.0))
+        qs = float(np.float32(2 * np.float32(1.4142135623730951)) / (np.float32(1.0) / snr))
+        frames.append(oc.demap(sym, 2, None, np.array([0, 1, 0, 0, 0, 0, 0, 0], np.uint32), code.N, qs, 0, 0)[0])
+    frames.append(_rail_frame(fs, rate, code.N, 31, code.K))
+    results = []
+    for f in frames:
+        h_o, r_o = code.decode(f[None], T)
+        h_n, r_n = _np_layered(name, f, T)
+        assert r_o[0] == r_n
+        np.testing.assert_array_equal(h_o[0], h_n)
+        results.append(r_n)
+    assert 0 <= results[0] < T and results[1] == -1 and 0 <= results[2] < T, results
+
+
 @pytest.mark.parametrize("fs,rate,name", [(0, r, n) for r, n in
                                           [("1/4", "s2_b1"), ("1/2", "s2_b4"), ("3/4", "s2_b7"),
                                            ("9/10", "s2_b11")]] +
