@@ -8,14 +8,16 @@ import pytest
 from cusat_amd import fer_sweep as fs
 
 
-def test_wilson_interval_known_values():
-    # 50 errors in 1000 frames: the textbook 95 % Wilson interval
-    lo, hi = fs.wilson(50, 1000)
-    assert lo == pytest.approx(0.03811, abs=2e-5) and hi == pytest.approx(0.06527, abs=2e-5)
-    lo, hi = fs.wilson(0, 2_000_000)
-    assert lo == 0.0 and hi == pytest.approx(1.92e-6, rel=1e-2)  # z^2 / (n + z^2)
+def test_wilson_interval_against_scipy():
+    from scipy.stats import binomtest
+    for k, n in ((50, 1000), (0, 2_000_000), (1000, 1_000_000), (3, 10), (10, 10)):
+        ref = binomtest(k, n).proportion_ci(confidence_level=0.95, method="wilson")
+        lo, hi = fs.wilson(k, n)
+        assert lo == pytest.approx(ref.low, rel=1e-6, abs=1e-12), (k, n)
+        assert hi == pytest.approx(ref.high, rel=1e-6), (k, n)
+    # large n: close to the normal approximation
     lo, hi = fs.wilson(1000, 1_000_000)
-    assert lo < 1e-3 < hi and hi - lo == pytest.approx(2 * 1.96 * math.sqrt(1e-3 * (1 - 1e-3) / 1e6), rel=2e-2)
+    assert hi - lo == pytest.approx(2 * 1.96 * math.sqrt(1e-3 * (1 - 1e-3) / 1e6), rel=2e-2)
 
 
 def test_crossing_of_a_log_linear_waterfall():
