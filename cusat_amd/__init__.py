@@ -40,7 +40,7 @@ class FecInfo(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in (
         "n_ldpc", "k_ldpc", "k_bch", "n_bch", "bch_t", "bch_degree", "q", "n_steps", "lanes",
         "cnl", "lt_alloc", "n_mod", "xfec_len", "demap_len", "max_batch", "max_trials",
-        "n_collision_slots", "lds_bytes")]
+        "n_collision_slots", "lds_bytes")] + [("llr_dmax", ctypes.c_float), ("llr_gain_cap", ctypes.c_float)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

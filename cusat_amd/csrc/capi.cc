@@ -272,6 +272,8 @@ void fill_info(const cusat_fec_ctx* c, cusat_fec_info* o)
     o->max_batch = c->max_batch;
     o->max_trials = c->max_trials;
     o->n_collision_slots = c->geo.n_slots;
+    o->llr_dmax = c->dmax;
+    o->llr_gain_cap = (c->llr_target > 0.0f && c->dmax > 0.0f) ? 2.0f * c->llr_target / c->dmax : 0.0f;
     const int ks = c->geo.n_steps + (c->geo.n_steps & 1), ng = (int)c->geo.groups.size();
     o->lds_bytes = (!(c->flags & CUSAT_CTX_NO_SPLIT_KERNEL) && ldpc_split_available(c->cmax, c->exact, c->geo.lanes))
                        ? ldpc_lds_bytes_split(c->geo.K, c->geo.R, c->geo.n_slots, ks, ng, c->geo.lanes)

@@ -73,6 +73,9 @@ typedef struct cusat_fec_info {
     int max_batch, max_trials;
     int n_collision_slots;     /* LDS delta slots the schedule needs */
     int lds_bytes;             /* dynamic LDS of the decode kernel */
+    float llr_dmax;            /* demapper's loudest-bit distance d^2 (QPSK 2, 8PSK 4, APSK: max over
+                                  (point, bit) of the nearest opposite-bit d^2; demapper_kernel.cu:216-237) */
+    float llr_gain_cap;        /* 2 * 320 / llr_dmax: the most llr_gain the demapper uses (0: none) */
 } cusat_fec_info;
 
 const char* cusat_last_error(void);

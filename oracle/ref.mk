@@ -10,4 +10,6 @@ _ref/libref_host.so: ref_shim.cc
 	mkdir -p _ref
 	$(CXX) $(CXXFLAGS) -shared -o $@ ref_shim.cc \
 	    $(SRC)/modulation/constellation.cc $(SRC)/modulation/bit_interleaver.cc \
-	    $(SRC)/fec/bch_code.cc $(SRC)/fec/code_params.cc $(SRC)/ldpc_table_factory.cc
+	    $(SRC)/modulation/dd_snr.cc \
+	    $(SRC)/fec/bch_code.cc $(SRC)/fec/code_params.cc $(SRC)/ldpc_table_factory.cc \
+	    $(SRC)/bb_scrambler.cc $(SRC)/bbframe_parser.cc $(SRC)/bb_header.cc

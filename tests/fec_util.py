@@ -26,8 +26,49 @@ def snr_scales(es_n0_db):
     return float(nv), float(gain)
 
 
+def dmax_by_definition(P, n_mod, mod):
+    """demapper_kernel.cu:216-237 restated with numpy float32: QPSK 2, 8PSK
+    4, APSK the largest, over (point, bit), of the smallest squared distance
+    to a point with the other value of that bit."""
+    if mod == "QPSK":
+        return np.float32(2.0)
+    if mod == "8PSK":
+        return np.float32(4.0)
+    P = np.asarray(P, np.float32).reshape(-1, 2)
+    dr = (P[:, None, 0] - P[None, :, 0]).astype(np.float32)
+    di = (P[:, None, 1] - P[None, :, 1]).astype(np.float32)
+    d = (dr * dr + di * di).astype(np.float32)
+    lab = np.arange(len(P))
+    dmax = np.float32(0.0)
+    for b in range(n_mod):
+        m = 1 << (n_mod - 1 - b)
+        opp = ((lab[:, None] ^ lab[None, :]) & m) != 0
+        dmax = max(dmax, np.where(opp, d, np.float32(1e30)).min(1).max())
+    return np.float32(dmax)
+
+
+def reference_demap_scales(ctx, noise_var, llr_gain):
+    """The demapper scaling as the reference computes it, from the context's
+    constellation: QPSK 2 sqrt(2) / noise_var (demapper_kernel.cu:64); 8PSK
+    2 sin(pi/8) times the capped gain (:84-99); the cap 2 * 320 / dmax
+    (capped_llr_gain, :283-291)."""
+    f32 = np.float32
+    qs = f32(f32(2.0) * f32(1.4142135623730951)) / f32(noise_var)
+    pts, _ = ctx.constellation()
+    mod = ca.MODULATIONS[ctx.modulation]
+    cap = f32(f32(2.0) * f32(320.0)) / dmax_by_definition(pts, ctx.info["n_mod"], mod)
+    g = f32(llr_gain) if f32(llr_gain) < cap else cap
+    ps = f32(f32(2.0) * f32(0.38268343236508977)) * g
+    return float(qs), float(ps), float(g)
+
+
 def oracle_demap(ctx, sym, noise_var, llr_gain):
-    qs, ps, g = ctx.demap_scales(noise_var, llr_gain)
+    """The oracle's LLRs for the context's frames, with the demapper scaling
+    computed independently of the library (and checked against what the
+    library uses)."""
+    qs, ps, g = reference_demap_scales(ctx, noise_var, llr_gain)
+    got = ctx.demap_scales(noise_var, llr_gain)
+    assert [np.float32(v) for v in got] == [np.float32(v) for v in (qs, ps, g)], (got, (qs, ps, g))
     pts, col = ctx.constellation()
     formula = ctx.modulation in (0, 1)
     return oc.demap(sym, ctx.info["n_mod"], None if formula else pts, col, ctx.info["n_ldpc"],

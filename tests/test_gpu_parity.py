@@ -92,6 +92,23 @@ def test_s2x_apsk_rings(fs, rate, mod, es):
     assert (tr >= 0).sum() >= 6, "near threshold most frames converge"
 
 
+@pytest.mark.parametrize("fs,rate,mod,es", [(0, "3/4", "8PSK", 20.0), (0, "2/3", "16APSK", 22.0),
+                                             (0, "3/4", "32APSK", 25.0), (1, "5/6", "16APSK", 24.0)])
+def test_demap_gain_cap_active(fs, rate, mod, es):
+    """High SNR, where the demapper lowers llr_gain to 2 * 320 / dmax
+    (capped_llr_gain, demapper_kernel.cu:283-291): 8PSK cap 160, 16APSK 2/3
+    cap 349.8, against llr_gain = min(4 / N0, 1000) = 400 .. 1000.  LLRs
+    bit-exact vs the oracle with the scaling recomputed from the
+    reference's definition, then the decode."""
+    from tests.fec_util import reference_demap_scales
+    ctx = ca.FecContext(fs, rate, mod, max_trials=50, max_batch=4)
+    nv, gain = snr_scales(es)
+    _, _, g = reference_demap_scales(ctx, nv, gain)
+    assert g < gain, "the cap is active here"
+    tr, va, _, _ = _chain_parity(fs, rate, mod, es, 8, 71)
+    assert (tr >= 45).all() and (va == 1).all(), "frames this far above threshold converge at once"
+
+
 def test_sequential_schedule_flag():
     """CUSAT_CTX_SEQUENTIAL_SCHEDULE = the reference's CUSAT_LDPC_SEQ=1."""
     _chain_parity(0, "3/4", "8PSK", 8.0, 16, 3, flags=ca.CTX_SEQUENTIAL_SCHEDULE)

@@ -291,3 +291,154 @@ def test_dd_snr_restatement_equals_the_library_host_estimate():
     bad = zero.copy()
     bad[8, 0] = np.nan
     assert c(pts16.ctypes.data, 16, bad.ctypes.data, 64, 8) == dd_snr_host(bad, pts16, False) == 0.0
+
+
+def _ref_ext():
+    L = _ref()
+    i, vp, f = ctypes.c_int, ctypes.c_void_p, ctypes.c_float
+    L.ref_qpsk_snr_dd.restype = f
+    L.ref_qpsk_snr_dd.argtypes = [vp, i]
+    L.ref_constellation_snr_dd.restype = f
+    L.ref_constellation_snr_dd.argtypes = [i, i, i, vp, i, i]
+    L.ref_bb_scramble.argtypes = [vp, vp, i]
+    L.ref_crc8.argtypes = [vp, i]
+    L.ref_extract_gse_datafield.argtypes = [vp, i, vp, i]
+    L.ref_build_gse_bbframe.argtypes = [i, i, vp, i, vp]
+    return L
+
+
+def _dsp_snr():
+    L = ctypes.CDLL(os.path.join(os.path.dirname(ca.__file__), "libcusat_dsp.so"))
+    q = L._ZN9cusat_dsp11qpsk_snr_ddEPKSt7complexIfEi
+    q.restype, q.argtypes = ctypes.c_float, [ctypes.c_void_p, ctypes.c_int]
+    c = L._ZN9cusat_dsp20constellation_snr_ddEPKSt7complexIfEiS3_ii
+    c.restype = ctypes.c_float
+    c.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    return q, c
+
+
+@pytest.mark.parametrize("mod,rate", [("QPSK", "1/2"), ("8PSK", "3/4"), ("16APSK", "2/3"), ("32APSK", "3/4"),
+                                      ("64APSK", "4/5"), ("128APSK", "135/180"), ("256APSK", "135/180"),
+                                      ("256APSK", "20/30"), ("8APSK", "100/180")])
+def test_dd_snr_host_estimate_vs_reference(mod, rate):
+    """libcusat_dsp's decision-directed SNR estimates (the chain's DD re-
+    estimate) equal the reference's own modulation/dd_snr.cc, compiled from
+    its source, bit for bit: QPSK over every symbol with sign decisions,
+    other constellations over every 8th symbol with the nearest point; noisy
+    frames at three noise levels, all-zero, NaN, inf and huge symbols."""
+    R = _ref_ext()
+    q, c = _dsp_snr()
+    pts = ca.modcod_constellation(0, rate, mod)[0].reshape(-1, 2).astype(np.float32)
+    mid, rid = ca.mod_id(mod), ca.rate_id(rate)
+    rng = np.random.default_rng(12)
+    n = 4001
+    cases = []
+    for sigma in (0.03, 0.2, 1.5):
+        cases.append((pts[rng.integers(0, len(pts), n)] + rng.normal(0, sigma, (n, 2))).astype(np.float32))
+    cases.append(np.zeros((n, 2), np.float32))
+    bad = cases[0].copy()
+    bad[16, 1] = np.nan
+    cases.append(bad)
+    big = cases[1].copy()
+    big[8, 0] = np.inf
+    cases.append(big)
+    cases.append((cases[1] * np.float32(1e19)).astype(np.float32))
+    for sym in cases:
+        sym = np.ascontiguousarray(sym)
+        if mod == "QPSK":
+            got, want = q(sym.ctypes.data, n), R.ref_qpsk_snr_dd(sym.ctypes.data, n)
+        else:
+            got = c(pts.ctypes.data, len(pts), sym.ctypes.data, n, 8)
+            want = R.ref_constellation_snr_dd(mid, rid, 0, sym.ctypes.data, n, 8)
+        assert np.float32(got).view(np.uint32) == np.float32(want).view(np.uint32), (got, want)
+
+
+@pytest.mark.parametrize("mod,rate", MODCODS)
+def test_demapper_dmax_and_gain_cap_vs_reference(mod, rate):
+    """The demapper's loudest-bit distance and capped gain
+    (capped_llr_gain, demapper_kernel.cu:216-239,283-291) recomputed from the
+    reference's own constellation (oracle/_ref) by the reference's
+    definition equal what the library uses (cusat_fec_query)."""
+    L = _ref()
+    rp = np.zeros(512, np.float32)
+    bits = ctypes.c_int()
+    assert L.ref_constellation(ca.mod_id(mod), ca.rate_id(rate), 0, rp.ctypes.data_as(ctypes.c_void_p),
+                               ctypes.byref(bits)) == 0
+    P = rp[: 2 << bits.value].reshape(-1, 2)
+    from tests.fec_util import dmax_by_definition
+    want = dmax_by_definition(P, bits.value, mod)
+    g = ca.geometry(0, rate, mod)
+    assert np.float32(g["llr_dmax"]).view(np.uint32) == want.view(np.uint32), (g["llr_dmax"], want)
+    cap = np.float32(np.float32(2.0) * np.float32(320.0)) / want
+    assert np.float32(g["llr_gain_cap"]).view(np.uint32) == np.float32(cap).view(np.uint32)
+    if (mod, rate) == ("16APSK", "2/3"):
+        assert abs(float(want) - 1.8295) < 1e-3 and abs(float(cap) - 349.8) < 0.1
+
+
+def test_bb_scrambler_vs_reference():
+    """The PRBS the library descrambles with equals the reference's
+    bb_scrambler.cc over a whole BBFRAME (7274 bytes), and scrambling is XOR."""
+    R = _ref_ext()
+    n = 7274
+    zero = np.zeros(n, np.uint8)
+    seq = np.zeros(n, np.uint8)
+    assert R.ref_bb_scramble(zero.ctypes.data, seq.ctypes.data, n) == 0
+    np.testing.assert_array_equal(ca.bb_prbs(n), seq)
+    data = np.random.default_rng(3).integers(0, 256, n, dtype=np.uint8)
+    out = np.zeros(n, np.uint8)
+    R.ref_bb_scramble(data.ctypes.data, out.ctypes.data, n)
+    np.testing.assert_array_equal(out, data ^ seq)
+
+
+def test_crc8_vs_reference():
+    R = _ref_ext()
+    rng = np.random.default_rng(5)
+    for n in (0, 1, 9, 10, 37, 255):
+        d = rng.integers(0, 256, n, dtype=np.uint8)
+        assert ca.bb_crc8(d) == R.ref_crc8(d.ctypes.data, n)
+
+
+def test_bbframe_parser_restatement_vs_reference():
+    """The data-field extraction the GPU tests check against
+    (tests/bb_ref.extract) equals the reference's bbframe_parser.cc
+    (decode_bbheader + extract_gse_datafield) on valid headers, flipped
+    header bits, DFLs that are not byte multiples or too large, the other
+    TS/GS values and DFL 0."""
+    from tests import bb_ref
+    R = _ref_ext()
+    rng = np.random.default_rng(8)
+    for kbch in (2001, 6051, 7274, 1332):
+        for trial in range(120):
+            kind = trial % 6
+            n_pay = int(rng.integers(0, kbch - 9))
+            f = bb_ref.make_bbframe(kbch, rng.integers(0, 256, n_pay, dtype=np.uint8), rng=rng)
+            if kind == 1:
+                f[rng.integers(0, 10)] ^= 1 << rng.integers(0, 8)  # CRC now fails
+            elif kind in (2, 3, 4, 5):
+                dfl = {2: 8 * n_pay + int(rng.integers(1, 8)), 3: 8 * kbch - 72, 4: 8 * n_pay, 5: 0}[kind]
+                f[4], f[5] = (dfl >> 8) & 0xFF, dfl & 0xFF
+                if kind == 4:
+                    f[0] = (f[0] & 0x3F) | (int(rng.integers(1, 4)) << 6)
+                f[9] = bb_ref.crc8(f[:9])
+            ext = bb_ref.extract(f, kbch)
+            out = np.zeros(kbch, np.uint8)
+            n = R.ref_extract_gse_datafield(f.ctypes.data, kbch, out.ctypes.data, kbch - 10)
+            assert n == (len(ext) if ext else 0), (kbch, trial, kind)
+            if n:
+                assert out[:n].tobytes() == ext
+
+
+@pytest.mark.parametrize("fs,rate", [(0, "1/4"), (0, "1/2"), (0, "3/4"), (0, "9/10"), (0, "135/180"),
+                                     (1, "1/2"), (1, "8/9")])
+def test_bench_bbframe_builder_vs_reference(fs, rate):
+    """The BBFRAMEs the benchmark sends (stream_bench.bbheader + a full data
+    field) equal what the reference transmitter builds (bb_header.cc
+    build_gse_bbframe: CCM, roll-off 0.20)."""
+    from cusat_amd import stream_bench
+    R = _ref_ext()
+    kbch = ca.geometry(fs, rate, "QPSK")["k_bch"] // 8
+    data = np.random.default_rng(2).integers(0, 256, kbch - 10, dtype=np.uint8)
+    out = np.zeros(kbch, np.uint8)
+    assert R.ref_build_gse_bbframe(fs, ca.rate_id(rate), data.ctypes.data, kbch - 10, out.ctypes.data) == kbch - 10
+    np.testing.assert_array_equal(out[:10], stream_bench.bbheader(kbch))
+    np.testing.assert_array_equal(out[10:], data)
