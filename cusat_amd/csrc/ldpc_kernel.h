@@ -172,6 +172,14 @@ __device__ __forceinline__ int wg_any(int v, int* flags, int& k)
 // VN group's first pair dword, number of slots, the slots' dword indices.
 constexpr int GROUP_WORDS = 8;
 
+// Iteration 0's old messages are zero (ldpc_decode_kernel.cu:495-581).  Its
+// record prefetches read this never-written, zero-initialised array instead
+// of the frame's records: the load stays unconditional (a pointer select,
+// no branch and no extra wait), the records need no clearing pass and
+// iteration 0 moves no record traffic.  Sized for the largest record of a
+// layer (HALF pairs x 4 uint4).
+static __device__ uint4 zero_records[HALF * 4];
+
 // word 3 of a raw buffer resource: 32-bit data format, no swizzle / stride;
 // loads beyond num_records bytes return 0
 constexpr int BUF_RSRC_WORD3 = 0x00020000;
@@ -227,19 +235,10 @@ __device__ __forceinline__ int decode_frame(unsigned char* lds_raw, const int16_
     }
     __syncthreads();
 
-    // Every thread zeroes its own message records (iteration 0 then reads
-    // old messages = 0 like any other iteration; each record is only ever
-    // touched by its owner thread, so program order orders these stores
-    // before the loads).
+    // Message records are not cleared: iteration 0 reads zero_records
+    // instead, and every record is written by its owner thread before it is
+    // first read (each layer once per iteration).
     const int tl = act ? tt : HALF - 1;  // idle lanes load a valid record they ignore
-    for (int s = 0; s < p.n_steps; s++) {
-        const int i = __builtin_amdgcn_readfirstlane(sched_l[2 * s + lane]);
-        if (act && i >= 0) {
-            uint4* dst = rec_base + ((size_t)i * HALF + tt) * R::V4;
-#pragma unroll
-            for (int k = 0; k < R::V4; k++) dst[k] = make_uint4(0, 0, 0, 0);
-        }
-    }
     // Double-buffered message records: even steps read ra and prefetch the
     // next step into rb, odd steps the reverse (the host pads the schedule
     // to an even step count).  The prefetch is unconditional, so the only
@@ -259,14 +258,14 @@ __device__ __forceinline__ int decode_frame(unsigned char* lds_raw, const int16_
             if (alpha) a = __builtin_amdgcn_raw_buffer_load_b32(rs_alpha, (itx * q + ix) * 4, 0, 0);
         }
     };
+    static_assert(R::V4 <= 4, "a layer's records fit zero_records");
     uint4 ra[R::V4], rb[R::V4];
     int pb[2], pa[2];  // beta, alpha of the step whose record is in ra / rb
     {
         const int i0 = max(__builtin_amdgcn_readfirstlane(sched_l[lane]), 0);
-        const uint4* src = rec_base + ((size_t)i0 * HALF + tl) * R::V4;
 #pragma unroll
         for (int k = 0; k < R::V4; k++) {
-            ra[k] = src[k];
+            ra[k] = make_uint4(0, 0, 0, 0);  // step 0 of iteration 0
             rb[k] = make_uint4(0, 0, 0, 0);
         }
         par_load(0, i0, pb[0], pa[0]);
@@ -326,10 +325,12 @@ __device__ __forceinline__ int decode_frame(unsigned char* lds_raw, const int16_
             {
                 const int sn = s + 1 < p.n_steps ? s + 1 : 0;
                 const int inx = max(__builtin_amdgcn_readfirstlane(sched_l[2 * sn + lane]), 0);
-                const uint4* src = rec_base + ((size_t)inx * HALF + tl) * R::V4;
+                const int itn = s + 1 < p.n_steps ? it : it + 1;  // iteration of the next step
+                const uint4* lay = itn > 0 ? rec_base + (size_t)inx * HALF * R::V4 : zero_records;
+                const uint4* src = lay + tl * R::V4;
 #pragma unroll
                 for (int k = 0; k < R::V4; k++) nxt[k] = src[k];
-                par_load(s + 1 < p.n_steps ? it : it + 1, inx, pb[hs ^ 1], pa[hs ^ 1]);
+                par_load(itn, inx, pb[hs ^ 1], pa[hs ^ 1]);
             }
             if (act && i >= 0) {
                 const int bt = __builtin_amdgcn_readfirstlane(pb[hs]);

@@ -126,16 +126,11 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
     }
     __syncthreads();
 
+    // Message records are not cleared: iteration 0 reads zero_records
+    // instead, and every record is written before it is first read (each
+    // layer once per iteration).
     const int tl = act ? tt : HALF - 1;
     auto rec_at = [&](int i, int pair) { return rec_base + (((size_t)i * 2 + half) * HALF + pair) * R::V4; };
-    for (int s = 0; s < p.n_steps; s++) {
-        const int i = __builtin_amdgcn_readfirstlane(sched_l[2 * s + lane].x);
-        if (act && i >= 0) {
-            uint4* dst = rec_at(i, tt);
-#pragma unroll
-            for (int k = 0; k < R::V4; k++) dst[k] = make_uint4(0, 0, 0, 0);
-        }
-    }
     // Learned parameters of a (iteration, layer): vector buffer loads issued
     // one step ahead with the message-record prefetch (a scalar load after
     // the step's barrier would stall every wave of the workgroup); loads
@@ -153,6 +148,7 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
             if (alpha) a = __builtin_amdgcn_raw_buffer_load_b32(rs_alpha, (itx * q + ix) * 4, 0, 0);
         }
     };
+    static_assert(R::V4 <= 4, "a layer's records fit zero_records");
     uint4 ra[R::V4], rb[R::V4];
     int pb[2], pa[2];  // beta, alpha of the step whose record is in ra / rb
     int si[2], so[2], sc[2], sg0[2];  // its layer, lay_off, circulant count, first group
@@ -167,10 +163,9 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
         sc[1] = 0;
         sg0[1] = 0;
         const int i0 = max(si[0], 0);
-        const uint4* src = rec_at(i0, tl);
 #pragma unroll
         for (int k = 0; k < R::V4; k++) {
-            ra[k] = src[k];
+            ra[k] = make_uint4(0, 0, 0, 0);  // step 0 of iteration 0
             rb[k] = make_uint4(0, 0, 0, 0);
         }
         par_load(0, i0, pb[0], pa[0]);
@@ -244,10 +239,12 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                 sc[hs ^ 1] = __builtin_amdgcn_readfirstlane(en.z);
                 sg0[hs ^ 1] = __builtin_amdgcn_readfirstlane(en.w);
                 const int inx = max(si[hs ^ 1], 0);
-                const uint4* src = rec_at(inx, tl);
+                const int itn = s + 1 < p.n_steps ? it : it + 1;  // iteration of the next step
+                const uint4* lay = itn > 0 ? rec_base + ((size_t)inx * 2 + half) * HALF * R::V4 : zero_records;
+                const uint4* src = lay + tl * R::V4;
 #pragma unroll
                 for (int k = 0; k < R::V4; k++) nxt[k] = src[k];
-                par_load(s + 1 < p.n_steps ? it : it + 1, inx, pb[hs ^ 1], pa[hs ^ 1]);
+                par_load(itn, inx, pb[hs ^ 1], pa[hs ^ 1]);
             }
             const bool work = act && i >= 0;
             const s2 zero = splat(0);
