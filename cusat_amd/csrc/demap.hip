@@ -205,7 +205,7 @@ __global__ void __launch_bounds__(256) demap_kernel(demap_args a)
 // consecutive sorted symbols.  Which lane demaps which symbol changes only
 // the speed; the values are demap_kernel's.
 
-constexpr int SORT_THREADS = 1024;
+constexpr int SORT_THREADS = DEMAP_BLOCK;
 constexpr int GRID_CELLS = 1024;  // 32 x 32
 
 __device__ __forceinline__ unsigned spread5(unsigned v)
@@ -239,16 +239,24 @@ __device__ __forceinline__ void demap_frame_sorted(sort_lds& L, int n_mod, const
                                                    const float2* __restrict__ y, int n, int N, float gain,
                                                    float scale, const uint32_t* col, int16_t* __restrict__ llr)
 {
-    static_assert(GRID_CELLS == SORT_THREADS, "one grid cell per thread in the scan");
+    constexpr int PER = GRID_CELLS / SORT_THREADS;  // cells per thread in the scan
+    static_assert(PER * SORT_THREADS == GRID_CELLS, "whole cells per thread");
     const int t = threadIdx.x;
-    L.cell[t] = 0;
+#pragma unroll
+    for (int k = 0; k < PER; k++) L.cell[PER * t + k] = 0;
     __syncthreads();
     for (int j = t; j < n; j += SORT_THREADS) atomicAdd(&L.cell[grid_cell(y[j], scale)], 1);
     __syncthreads();
-    // exclusive scan of the counts: within each wave, then over the waves
-    const int v = L.cell[t];
+    // exclusive scan of the counts: PER per thread, within each wave, then
+    // over the waves
+    int v[PER], run = 0;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        v[k] = L.cell[PER * t + k];
+        run += v[k];
+    }
     const int lane = t & 63, w = t >> 6;
-    int inc = v;
+    int inc = run;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
         const int o = __shfl_up(inc, d, 64);
@@ -256,9 +264,13 @@ __device__ __forceinline__ void demap_frame_sorted(sort_lds& L, int n_mod, const
     }
     if (lane == 63) L.wave_sum[w] = inc;
     __syncthreads();
-    int base = inc - v;
+    int base = inc - run;
     for (int k = 0; k < w; k++) base += L.wave_sum[k];
-    L.cell[t] = base;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        L.cell[PER * t + k] = base;
+        base += v[k];
+    }
     __syncthreads();
     for (int j = t; j < n; j += SORT_THREADS) {
         const int p = atomicAdd(&L.cell[grid_cell(y[j], scale)], 1);
@@ -282,46 +294,48 @@ __global__ void __launch_bounds__(SORT_THREADS) demap_sorted_kernel(demap_args a
                        a.gain, a.grid_scale, a.col, a.llr + (size_t)f * a.N);
 }
 
-// Mixed-MODCOD batch, frames of the sorted path: a fixed grid takes them in
-// dlist order (hdr[3] of them, read from device memory so a captured launch
-// serves any batch).
+// Mixed-MODCOD batch: a fixed grid takes work items from a queue in device
+// memory (hdr[8], reset by the last workgroup out through hdr[9], so a
+// captured launch replays any plan).  Items 0 .. hdr[3] - 1 are the frames
+// of the sorted path in dlist order, one workgroup each; the next hdr[1]
+// are DEMAP_BLOCK-symbol blocks of the other frames (frame dlist[k].x owns
+// blocks dlist[k].y .. dlist[k + 1].y - 1).  Both parts are in descending
+// cost, so the queue hands out the heavy work first and the tail is cheap.
+// Every symbol gets exactly the arithmetic of demap_kernel.
 __global__ void __launch_bounds__(SORT_THREADS)
-    demap_mixed_sorted_kernel(const rx_code* __restrict__ codes, const rx_rec* __restrict__ recs,
-                              const int2* __restrict__ dlist, const rx_scale* __restrict__ snap,
-                              const int* __restrict__ hdr, const float* __restrict__ sym, int16_t* __restrict__ llr)
+    demap_mixed_kernel(const rx_code* __restrict__ codes, const rx_rec* __restrict__ recs,
+                       const int2* __restrict__ dlist, const rx_scale* __restrict__ snap, int* hdr,
+                       const float* __restrict__ sym, int16_t* __restrict__ llr)
 {
     __shared__ sort_lds L;
-    const int nh = hdr[3];
-    for (int k = blockIdx.x; k < nh; k += gridDim.x) {
-        const int f = dlist[k].x;
-        const rx_rec& r = recs[f];
-        const rx_code& c = codes[r.code];
-        const float gain = (r.flags & RX_SCALE_STATE) ? snap[r.code].gain : r.gain;
-        uint32_t col[8];
+    __shared__ int item_s;
+    const int n = hdr[0], nh = hdr[3];
+    const int total = nh + hdr[1];
+    unsigned* qc = reinterpret_cast<unsigned*>(hdr + 8);
+    int k_lo = nh;  // a workgroup's block items only grow: their frame only moves forward
+    for (;;) {
+        if (threadIdx.x == 0) item_s = (int)atomicAdd(&qc[0], 1u);
+        __syncthreads();
+        const int item = __builtin_amdgcn_readfirstlane(item_s);
+        __syncthreads();
+        if (item >= total) break;
+        if (item < nh) {
+            const int f = dlist[item].x;
+            const rx_rec& r = recs[f];
+            const rx_code& c = codes[r.code];
+            const float gain = (r.flags & RX_SCALE_STATE) ? snap[r.code].gain : r.gain;
+            uint32_t col[8];
 #pragma unroll
-        for (int b = 0; b < 8; b++) col[b] = c.col[b];
-        demap_frame_sorted(L, c.n_mod, c.pts, reinterpret_cast<const float2*>(sym) + r.sym_off, c.dlen, c.N, gain,
-                           c.grid_scale, col, llr + r.llr_off);
-    }
-}
-
-// Mixed-MODCOD batch, the other frames: a fixed grid walks their 256-symbol
-// blocks in dlist order (frame dlist[k].x owns blocks dlist[k].y ..
-// dlist[k + 1].y - 1; hdr[1] blocks in all, all read from device memory).
-// The frames are in descending cost, so the heavy blocks go first and the
-// tail is cheap.  Every symbol gets exactly the arithmetic of demap_kernel.
-__global__ void __launch_bounds__(256)
-    demap_mixed_kernel(const rx_code* __restrict__ codes, const rx_rec* __restrict__ recs,
-                       const int2* __restrict__ dlist, const rx_scale* __restrict__ snap,
-                       const int* __restrict__ hdr, const float* __restrict__ sym, int16_t* __restrict__ llr)
-{
-    const int n = hdr[0], total = hdr[1], nh = hdr[3];
-    int k_lo = nh;  // blocks are taken in ascending order: the frame only moves forward
-    for (int item = blockIdx.x; item < total; item += gridDim.x) {
-        int lo = k_lo, hi = n - 1;  // last entry with first block <= item
+            for (int b = 0; b < 8; b++) col[b] = c.col[b];
+            demap_frame_sorted(L, c.n_mod, c.pts, reinterpret_cast<const float2*>(sym) + r.sym_off, c.dlen, c.N,
+                               gain, c.grid_scale, col, llr + r.llr_off);
+            continue;
+        }
+        const int blk = item - nh;
+        int lo = k_lo, hi = n - 1;  // last entry with first block <= blk
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if (dlist[mid].y <= item) lo = mid;
+            if (dlist[mid].y <= blk) lo = mid;
             else hi = mid - 1;
         }
         k_lo = lo;
@@ -347,8 +361,16 @@ __global__ void __launch_bounds__(256)
             a.psk8_scale = r.psk8_scale;
             a.gain = r.gain;
         }
-        const int j = (item - e.y) * 256 + threadIdx.x;
+        const int j = (blk - e.y) * DEMAP_BLOCK + threadIdx.x;
         if (j < a.dlen) demap_symbol(a, j, reinterpret_cast<const float2*>(sym) + r.sym_off, llr + r.llr_off);
+    }
+    // the last workgroup to leave resets the queue for the next launch
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&qc[1], 1u) == gridDim.x - 1) {
+            atomicExch(&qc[0], 0u);
+            atomicExch(&qc[1], 0u);
+        }
     }
 }
 
@@ -525,14 +547,10 @@ hipError_t launch_demap(const demap_args& a, int batch, hipStream_t stream)
 }
 
 hipError_t launch_demap_mixed(const rx_code* codes, const rx_rec* recs, const int2* dlist, const rx_scale* snap,
-                              const int* hdr, const float* sym, int16_t* llr, int n_cu, hipStream_t stream)
+                              int* hdr, const float* sym, int16_t* llr, int n_cu, hipStream_t stream)
 {
-    hipLaunchKernelGGL(demap_mixed_sorted_kernel, dim3(n_cu * 4), dim3(SORT_THREADS), 0, stream, codes, recs, dlist,
-                       snap, hdr, sym, llr);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(demap_mixed_kernel, dim3(n_cu * 8), dim3(256), 0, stream, codes, recs, dlist, snap, hdr, sym,
-                       llr);
+    hipLaunchKernelGGL(demap_mixed_kernel, dim3(n_cu * 4), dim3(SORT_THREADS), 0, stream, codes, recs, dlist, snap,
+                       hdr, sym, llr);
     return hipGetLastError();
 }
 

@@ -76,6 +76,7 @@ hipError_t launch_demap(const demap_args& a, int batch, hipStream_t stream);
 // tables of at least SORTED_DEMAP_MIN_POINTS points and at most
 // SORTED_DEMAP_MAX_SYMBOLS demapped symbols.
 constexpr int SORTED_DEMAP_MIN_POINTS = 64;
+constexpr int DEMAP_BLOCK = 512;  // symbols per work item of the other frames of a mixed batch
 constexpr int SORTED_DEMAP_MAX_SYMBOLS = 16200;
 inline bool demap_sorted(bool table, int n_points, int dlen)
 {
@@ -120,14 +121,16 @@ struct rx_event {
 struct rx_scale {
     float qpsk_scale, psk8_scale, gain, snr_lin;
 };
-// hdr[0] = frames, hdr[1] = demap blocks of the block-walk frames, hdr[2] =
-// SNR events, hdr[3] = frames of the sorted path (device memory).  dlist:
-// per frame (frame number, first block): the sorted-path frames first,
-// costliest first, then the block-walk frames, costliest first (blocks
-// numbered in that order).  Two launches: the sorted-path frames, one
-// workgroup per frame, then the 256-symbol blocks of the others.
+// hdr[0] = frames, hdr[1] = DEMAP_BLOCK-symbol blocks of the frames outside
+// the sorted path, hdr[2] = SNR events, hdr[3] = frames of the sorted path,
+// hdr[8, 9] = the demap launch's work queue and exit count (0 in a fresh
+// plan; the launch leaves them 0) -- device memory.  dlist: per frame
+// (frame number, first block): the sorted-path frames first, costliest
+// first, then the others, costliest first (blocks numbered in that order).
+// One launch: a fixed grid takes work items from the queue -- the
+// sorted-path frames one workgroup each, then the blocks of the others.
 hipError_t launch_demap_mixed(const rx_code* codes, const rx_rec* recs, const int2* dlist, const rx_scale* snap,
-                              const int* hdr, const float* sym, int16_t* llr, int n_cu, hipStream_t stream);
+                              int* hdr, const float* sym, int16_t* llr, int n_cu, hipStream_t stream);
 // est[e] = decision-directed SNR of the frame of every estimate event
 hipError_t launch_dd_estimate(const rx_code* codes, const rx_rec* recs, const rx_event* ev, const int* hdr,
                               const float* sym, float* est, int grid, hipStream_t stream);

@@ -966,7 +966,7 @@ extern "C" int cusat_stream_rx_plan_async(cusat_stream_ctx* sc, const cusat_stre
                 n_sorted++;
             } else {
                 dl[k] = make_int2(f, blk);
-                blk += (sc->codes[frames[f].code]->dlen + 255) / 256;
+                blk += (sc->codes[frames[f].code]->dlen + DEMAP_BLOCK - 1) / DEMAP_BLOCK;
             }
         }
     }
@@ -1050,7 +1050,7 @@ extern "C" int cusat_stream_rx_demap(cusat_stream_ctx* sc, const float* sym, voi
     if (!sc || !sym) { set_last_error("bad argument"); return CUSAT_E_INVALID; }
     const int rc = need_rx_plan(sc, "cusat_stream_rx_demap");
     if (rc != CUSAT_OK) return rc;
-    const int* hdr = rx_header(sc);
+    int* hdr = const_cast<int*>(rx_header(sc));
     const rx_rec* rec = reinterpret_cast<const rx_rec*>(hdr + 64);
     const int2* dlist =
         reinterpret_cast<const int2*>(reinterpret_cast<const unsigned char*>(sc->d_counters) + sc->off_dl);
