@@ -99,6 +99,8 @@ def lib():
         L.cusat_stream_plan_async.argtypes = [vp, vp, i, vp]
         L.cusat_stream_rx_plan_async.argtypes = [vp, vp, i, vp, i, vp]
         L.cusat_stream_rx_dd.argtypes = [vp, vp, vp, vp, vp]
+        L.cusat_stream_rx_dd_estimate.argtypes = [vp, vp, vp]
+        L.cusat_stream_rx_dd_resolve.argtypes = [vp, vp, vp, vp]
         L.cusat_stream_rx.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.cusat_stream_rx_demap.argtypes = [vp, vp, vp]
         L.cusat_stream_rx_decode.argtypes = [vp, vp, vp, vp, vp]
@@ -306,6 +308,15 @@ class StreamContext:
         """Apply the batch's SNR events to state (float32 [n_codes, 4], device)."""
         _check(lib().cusat_stream_rx_dd(self._h, _ptr(sym), _ptr(state), _ptr(state_out), _stream(stream)),
                "cusat_stream_rx_dd")
+
+    def rx_dd_estimate(self, sym, stream=None):
+        """The estimate launch of rx_dd (reads the symbols and the plan only)."""
+        _check(lib().cusat_stream_rx_dd_estimate(self._h, _ptr(sym), _stream(stream)), "cusat_stream_rx_dd_estimate")
+
+    def rx_dd_resolve(self, state, state_out=None, stream=None):
+        """The resolve launch of rx_dd: the events applied in order to state."""
+        _check(lib().cusat_stream_rx_dd_resolve(self._h, _ptr(state), _ptr(state_out), _stream(stream)),
+               "cusat_stream_rx_dd_resolve")
 
     def rx(self, sym, out, trials, valid, dfl, stream=None):
         """Demap, decode, BCH-check and BB-extract the current receive batch."""

@@ -415,10 +415,13 @@ __global__ void __launch_bounds__(256)
         const int stride = qpsk ? 1 : 8;
         const int m = (c.xfec + stride - 1) / stride;
         float acc = 0.0f;  // lane 0: signal, lane 1: error
+        // the next chunk's symbol is loaded before this chunk is summed, so
+        // its latency hides behind the two lanes' adds
+        float2 next = t < m ? y[(size_t)t * stride] : make_float2(0.0f, 0.0f);
         for (int c0 = 0; c0 < m; c0 += 256) {
             const int j = c0 + t;
+            const float2 v = next;
             if (j < m) {
-                const float2 v = y[(size_t)j * stride];
                 float sig, e1, e2;
                 if (qpsk) {
                     const float a = 0.70710678118654752f;
@@ -450,6 +453,7 @@ __global__ void __launch_bounds__(256)
                 term_b[1][t] = e2;
             }
             __syncthreads();
+            if (j + 256 < m) next = y[(size_t)(j + 256) * stride];
             if (t < 2) {
                 const int n = min(256, m - c0);
                 const float* A = term_a[t];

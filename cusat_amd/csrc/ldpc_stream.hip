@@ -1100,21 +1100,39 @@ extern "C" int cusat_stream_rx_work(const cusat_stream_ctx* sc, const int16_t** 
     return CUSAT_OK;
 }
 
-extern "C" int cusat_stream_rx_dd(cusat_stream_ctx* sc, const float* sym, float* state, float* state_out,
-                                  void* stream)
+extern "C" int cusat_stream_rx_dd_estimate(cusat_stream_ctx* sc, const float* sym, void* stream)
 {
-    if (!sc || !sym || !state) { set_last_error("bad argument"); return CUSAT_E_INVALID; }
-    const int rc = need_rx_plan(sc, "cusat_stream_rx_dd");
+    if (!sc || !sym) { set_last_error("bad argument"); return CUSAT_E_INVALID; }
+    const int rc = need_rx_plan(sc, "cusat_stream_rx_dd_estimate");
     if (rc != CUSAT_OK) return rc;
-    hipStream_t st = (hipStream_t)stream;
     const unsigned char* base = reinterpret_cast<const unsigned char*>(sc->d_counters);
     const int* hdr = reinterpret_cast<const int*>(base + sc->off_rx);
     const rx_rec* rec = reinterpret_cast<const rx_rec*>(hdr + 64);
     const rx_event* ev = reinterpret_cast<const rx_event*>(base + sc->off_ev);
-    HIP_TRY(launch_dd_estimate(sc->d_rx_codes, rec, ev, hdr, sym, sc->d_est, 64, st));
-    HIP_TRY(launch_dd_resolve(sc->d_rx_codes, ev, sc->d_est, hdr, (int)sc->codes.size(),
-                              reinterpret_cast<float4*>(state), sc->d_snap, reinterpret_cast<float4*>(state_out), st));
+    HIP_TRY(launch_dd_estimate(sc->d_rx_codes, rec, ev, hdr, sym, sc->d_est, 64, (hipStream_t)stream));
     return CUSAT_OK;
+}
+
+extern "C" int cusat_stream_rx_dd_resolve(cusat_stream_ctx* sc, float* state, float* state_out, void* stream)
+{
+    if (!sc || !state) { set_last_error("bad argument"); return CUSAT_E_INVALID; }
+    const int rc = need_rx_plan(sc, "cusat_stream_rx_dd_resolve");
+    if (rc != CUSAT_OK) return rc;
+    const unsigned char* base = reinterpret_cast<const unsigned char*>(sc->d_counters);
+    const int* hdr = reinterpret_cast<const int*>(base + sc->off_rx);
+    const rx_event* ev = reinterpret_cast<const rx_event*>(base + sc->off_ev);
+    HIP_TRY(launch_dd_resolve(sc->d_rx_codes, ev, sc->d_est, hdr, (int)sc->codes.size(),
+                              reinterpret_cast<float4*>(state), sc->d_snap, reinterpret_cast<float4*>(state_out),
+                              (hipStream_t)stream));
+    return CUSAT_OK;
+}
+
+extern "C" int cusat_stream_rx_dd(cusat_stream_ctx* sc, const float* sym, float* state, float* state_out,
+                                  void* stream)
+{
+    if (!sc || !sym || !state) { set_last_error("bad argument"); return CUSAT_E_INVALID; }
+    const int rc = cusat_stream_rx_dd_estimate(sc, sym, stream);
+    return rc != CUSAT_OK ? rc : cusat_stream_rx_dd_resolve(sc, state, state_out, stream);
 }
 
 extern "C" int cusat_stream_autotune(cusat_stream_ctx* sc, const int16_t* llr, uint8_t* hard, int32_t* trials,
@@ -1129,7 +1147,10 @@ extern "C" int cusat_stream_autotune(cusat_stream_ctx* sc, const int16_t* llr, u
     cusat_stream_tuning best = sc->tuning;
     float best_t = 1e30f;
     int rc = CUSAT_OK;
-    for (int split = 0; split < 2 && rc == CUSAT_OK; split++)
+    // the 12-wave kernel has a form for every code and has been the faster
+    // one on every measured mix: the 6-wave kernel is timed only as the
+    // fallback when the 12-wave one does not fit
+    for (int split = sc->max_wg[1] >= 1 ? 1 : 0; split < 2 && rc == CUSAT_OK; split++)
     for (int order = 0; order < 2 && rc == CUSAT_OK; order++)
         for (int fpj : {1, 2, 4})
             for (int wg = 1; wg <= sc->max_wg[split] && rc == CUSAT_OK; wg++) {

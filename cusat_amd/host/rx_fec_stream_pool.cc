@@ -109,14 +109,16 @@ rx_fec_stream_pool::rx_fec_stream_pool(fecframe_size_t framesize, const std::vec
             int32_t* res = r.res_dev;
             const float* sym = reinterpret_cast<const float*>(r.sym_dev);
             auto rx = [&] { return cusat_stream_rx(r.sc, sym, r.out_dev, res, res + mb, res + 2 * mb, st); };
-            auto dd = [&] { return cusat_stream_rx_dd(r.sc, sym, d_state, r.state_dev, st); };
-            abi_check(dd(), "snr");
+            auto est = [&] { return cusat_stream_rx_dd_estimate(r.sc, sym, st); };
+            auto resolve = [&] { return cusat_stream_rx_dd_resolve(r.sc, d_state, r.state_dev, st); };
+            abi_check(est(), "snr");
+            abi_check(resolve(), "snr");
             abi_check(rx(), "rx");
             hip_check(hipStreamSynchronize(st), "warm launch");
-            for (int k = 0; k < 2; k++) {
+            for (int k = 0; k < 3; k++) {
                 hipGraph_t g = nullptr;
                 hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "capture");
-                const int rc = k ? rx() : dd();
+                const int rc = k == 0 ? est() : k == 1 ? resolve() : rx();
                 const hipError_t ec = hipStreamEndCapture(st, &g);
                 abi_check(rc, "capture");
                 hip_check(ec, "capture");
@@ -124,7 +126,7 @@ rx_fec_stream_pool::rx_fec_stream_pool(fecframe_size_t framesize, const std::vec
                 const hipError_t ei = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
                 hipGraphDestroy(g);
                 hip_check(ei, "graph instantiate");
-                (k ? r.graph : r.graph_dd) = ge;
+                (k == 0 ? r.graph_dd_est : k == 1 ? r.graph_dd : r.graph) = ge;
             }
         }
         // a ring of about N_FRAMES_BUFFERED frames (128 micro-batches of 4 in
@@ -173,6 +175,7 @@ void rx_fec_stream_pool::free_all()
     for (auto& r : d_streams) {
         if (r.graph) hipGraphExecDestroy(r.graph);
         if (r.graph_dd) hipGraphExecDestroy(r.graph_dd);
+        if (r.graph_dd_est) hipGraphExecDestroy(r.graph_dd_est);
         if (r.sc) cusat_stream_destroy(r.sc);
         if (r.sym_dev) hipFree(r.sym_dev);
         if (r.blk) hipFree(r.blk);
@@ -364,7 +367,9 @@ void rx_fec_stream_pool::launch_slot(flight_slot& s)
                   "copy in");
         abi_check(cusat_stream_rx_plan_async(r.sc, rec, n, s.events.data(), (int)s.events.size(), st), "plan");
         if (d_gpu_dd) {
-            // the SNR state passes from micro-batch to micro-batch in launch order
+            // the estimate needs only this micro-batch's symbols; the SNR
+            // state passes from micro-batch to micro-batch in launch order
+            hip_check(hipGraphLaunch(r.graph_dd_est, st), "graph launch");
             if (d_dd_recorded) hip_check(hipStreamWaitEvent(st, d_dd_done, 0), "wait");
             hip_check(hipGraphLaunch(r.graph_dd, st), "graph launch");
             hip_check(hipEventRecord(d_dd_done, st), "event");

@@ -24,12 +24,14 @@
  *    micro-batch is closed (a chain reads it when it launches one; the two
  *    agree whenever the budget lets a micro-batch launch at once).  The
  *    decision-directed SNR is refreshed every 32nd frame of that MODCOD
- *    (rx_fec_chain.cc:321-342).  By default the estimate runs on the GPU in
- *    the micro-batch's own launch sequence (cusat_stream_rx_dd: bit-identical
- *    sums, no host pass over the symbols); update_snr values travel with the
- *    micro-batch as events, in submit order; data_snr_db() follows once a
- *    micro-batch is drained.  set_gpu_dd_snr(false) estimates on the host at
- *    submit instead, as a chain does;
+ *    (rx_fec_chain.cc:321-342).  By default the estimate runs on the host
+ *    at submit, as a chain does.  set_gpu_dd_snr(true) runs it on the GPU
+ *    in the micro-batch's own launch sequence instead
+ *    (cusat_stream_rx_dd_estimate and _resolve: bit-identical sums, no host
+ *    pass over the symbols; only the resolve steps wait for the previous
+ *    micro-batch's); update_snr values then travel with the micro-batch as
+ *    events, in submit order, and data_snr_db() follows once a micro-batch
+ *    is drained;
  *  - micro_batch = frames per launch across all MODCODs; n_gpu_streams
  *    streams (each with its own stream context and captured graph) overlap
  *    copies and launches; the decode budget caps frames in flight as
@@ -103,8 +105,10 @@ public:
     uint64_t total_gpu_drop_count() const { return d_gpu_drop_cnt; }
     float data_snr_db() const;
 
-    /** true (default): decision-directed SNR on the GPU; false: on the host.
-     *  Set before the first submit. */
+    /** true: decision-directed SNR on the GPU; false (default): on the host,
+     *  at submit, as a chain does (the faster of the two on a 12-MODCOD
+     *  stream of 6144 frames: 128-151 k vs 112-126 k frames/s).  Set before
+     *  the first submit. */
     void set_gpu_dd_snr(bool on) { d_gpu_dd = on; }
     /** Wait for the oldest micro-batch instead of dropping a frame when the
      *  ring is full (rx_fec_chain::set_block_when_full). */
@@ -152,7 +156,8 @@ private:
         float* state_dev = nullptr;      // state after the stream's latest micro-batch
         uint8_t* out_dev = nullptr;
         hipGraphExec* graph = nullptr;   // cusat_stream_rx
-        hipGraphExec* graph_dd = nullptr;  // cusat_stream_rx_dd
+        hipGraphExec* graph_dd_est = nullptr;  // cusat_stream_rx_dd_estimate
+        hipGraphExec* graph_dd = nullptr;      // cusat_stream_rx_dd_resolve
     };
     struct block {
         std::vector<uint8_t> bytes;
@@ -176,7 +181,7 @@ private:
     std::chrono::steady_clock::time_point d_batch_start_time;
     std::mutex* d_mtx = nullptr;
     std::shared_ptr<gpu_decode_budget> d_budget;
-    bool d_gpu_dd = true;
+    bool d_gpu_dd = false;
     bool d_block_when_full = false;
     float* d_state = nullptr;                 // device SNR state, float4 per MODCOD
     ihipEvent_t* d_dd_done = nullptr;         // recorded after the latest SNR launch

@@ -328,6 +328,12 @@ int cusat_stream_rx_plan_async(cusat_stream_ctx* sc, const cusat_stream_rx_frame
  * snr_db NaN where an estimate set it.  Launches that share one state must
  * be ordered (the same stream, or events between streams).  Capturable. */
 int cusat_stream_rx_dd(cusat_stream_ctx* sc, const float* sym, float* state, float* state_out, void* stream);
+/* The two launches of cusat_stream_rx_dd.  The estimate reads only the
+ * symbols and the plan, so estimates of consecutive batches (on different
+ * stream contexts) may run concurrently; only the resolve steps that share
+ * a state must be ordered.  Capturable. */
+int cusat_stream_rx_dd_estimate(cusat_stream_ctx* sc, const float* sym, void* stream);
+int cusat_stream_rx_dd_resolve(cusat_stream_ctx* sc, float* state, float* state_out, void* stream);
 /* Run the current rx plan: sym = device symbols, out = device data-field
  * slots, trials / valid / dfl = B ints each by frame number (dfl as
  * cusat_bb_extract writes it: -1 FEC failure, 0 no data field, n bytes). */
@@ -354,9 +360,11 @@ int cusat_stream_rx_work(const cusat_stream_ctx* sc, const int16_t** llr, const 
  * frame number. */
 int cusat_stream_decode(cusat_stream_ctx* sc, const int16_t* llr, uint8_t* hard, int32_t* trials,
                         int32_t* valid, int32_t* fallback, void* stream);
-/* Time every tuning candidate (split_edges, workgroups per CU, frames per job, order)
- * over reps launches on the given buffers, keep the fastest (results do not
- * depend on the tuning) and report it with its time per launch. */
+/* Time every tuning candidate (workgroups per CU, frames per job, order) of
+ * the 12-wave kernel -- of the 6-wave kernel only when the 12-wave one does
+ * not fit the device -- over reps launches on the given buffers, keep the
+ * fastest (results do not depend on the tuning) and report it with its time
+ * per launch.  cusat_stream_plan still accepts split_edges 0. */
 int cusat_stream_autotune(cusat_stream_ctx* sc, const int16_t* llr, uint8_t* hard, int32_t* trials,
                           int32_t* valid, int32_t* fallback, int reps, void* stream,
                           cusat_stream_tuning* chosen, float* best_ms);

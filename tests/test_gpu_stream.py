@@ -126,6 +126,7 @@ def test_stream_autotune_keeps_results():
     va = torch.zeros(n, dtype=torch.int32, device="cuda")
     chosen, ms = sc.autotune(llr, hard, tr, va, reps=1)
     assert ms > 0 and chosen == {k: sc.tuning()[k] for k in chosen}
+    assert chosen["split_edges"] == 1  # the 6-wave kernel is only the fallback
     check(ctxs, frames, who, ref, run_stream(sc, frames, llr, hb))
 
 

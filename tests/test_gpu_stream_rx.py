@@ -236,3 +236,10 @@ def test_stream_rx_scaling_from_the_device_snr_state():
     check(frames, recs, ref, out, tr, va, dfl)
     # the device state itself was updated for the next batch
     assert np.array_equal(state.cpu().numpy(), st, equal_nan=True)
+    # the estimate and resolve launches on their own give the same state
+    state2 = torch.tensor([[1.0, 4.0, 0.0, 0.0]] * 3, dtype=torch.float32, device="cuda")
+    state2_out = torch.zeros_like(state2)
+    sc.rx_dd_estimate(d_sym)
+    sc.rx_dd_resolve(state2, state2_out)
+    torch.cuda.synchronize()
+    assert np.array_equal(state2_out.cpu().numpy(), st, equal_nan=True)
