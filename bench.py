@@ -57,10 +57,13 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--frames", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=20240612)
-    ap.add_argument("--cpu-reps", type=int, default=2, help="passes of the CPU baseline over the stream")
+    ap.add_argument("--cpu-reps", type=int, default=3, help="passes of the CPU baseline over the stream")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the secondary workloads")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--worst-case", action="store_true",
+                    help="run only the 8PSK 3/4 worst case on the batched decoder and print its JSON object")
+    ap.add_argument("--no-split", action="store_true", help="worst case on the 6-wave decoder kernel")
     return ap.parse_args()
 
 
@@ -78,6 +81,12 @@ def cpu_info():
     except AttributeError:
         avail = os.cpu_count() or 1
     return model, os.cpu_count() or 1, avail
+
+
+def cpu_threads(avail):
+    """Threads of the CPU baseline: every core this process may run on, up
+    to the 16 host cores a one-GPU job is given."""
+    return max(1, min(16, avail))
 
 
 def stream_step(sc, sym, out, tr, va, fb, dfl, ev=None):
@@ -211,7 +220,7 @@ def cpu_baseline(st, r, reps, nthr):
     return dt, same
 
 
-def plain_worst_case(nthr):
+def plain_worst_case(nthr, split=True, reps=5):
     """DVB-S2 normal 8PSK 3/4, 1024 frames at Es/N0 7.0 dB, T = 50 (every
     frame at the iteration cap), demap -> batched decode -> BCH; all 1024
     frames against the oracle."""
@@ -219,7 +228,7 @@ def plain_worst_case(nthr):
     import cusat_amd as ca
     from oracle import oracle_ctypes as oc
     fs, rate, mod, B, es = 0, "3/4", "8PSK", 1024, 7.0
-    ctx = ca.FecContext(fs, rate, mod, max_trials=T, max_batch=B)
+    ctx = ca.FecContext(fs, rate, mod, max_trials=T, max_batch=B, flags=0 if split else ca.CTX_NO_SPLIT_KERNEL)
     inf = ctx.info
     N, K, kb = inf["n_ldpc"], inf["k_ldpc"], inf["k_bch"]
     sym, _, _ = ca.gen_frames(fs, rate, mod, 20240607, B, es)
@@ -232,7 +241,6 @@ def plain_worst_case(nthr):
     for _ in range(2):
         ctx.rx_fec_batch(sym_d, llr, B, nv, gain, hard, tr, va)
     torch.cuda.synchronize()
-    reps = 5
     ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(reps)]
     t0 = time.perf_counter()
     for e in ev:
@@ -253,9 +261,12 @@ def plain_worst_case(nthr):
     h_o, t_o = oc.Code("s2_b7").decode(llr.cpu().numpy(), T, nthreads=nthr)
     traffic = None
     tj = os.path.join(ROOT, "profiles", "ldpc_kernel_traffic.json")
-    if os.path.exists(tj):
+    if os.path.exists(tj) and split:
         traffic = json.load(open(tj)).get("hbm_bytes_per_launch")
     return {"workload": "DVB-S2 normal 8PSK 3/4 plain, 1024 frames at 7.0 dB, demap + batched decode + BCH",
+            "kernel": "ldpc_kernel_split<12, true, 2>" if split else "ldpc_kernel<12, true, 2> (6-wave)",
+            "ms_bch": round(float(np.mean([e[2].elapsed_time(e[3]) for e in ev])), 4),
+            "ms_demap": round(float(np.mean([e[0].elapsed_time(e[1]) for e in ev])), 4),
             "gbps_all_frames": round(B * kb / (ms * 1e-3) / 1e9, 4), "ms_per_step": round(ms, 4),
             "ms_ldpc": round(ms_dec, 4), "mean_iterations": float(iters.mean()),
             "roofline": {"bound": "hbm", "achieved": round(alg / (ms_dec * 1e-3) / 1e9, 2), "peak": HBM_PEAK_GBS,
@@ -293,6 +304,10 @@ def main():
     import cusat_amd as ca
     from cusat_amd import stream_bench
 
+    if args.worst_case:
+        print(json.dumps(plain_worst_case(cpu_threads(cpu_info()[2]), split=not args.no_split,
+                                          reps=args.steps)), flush=True)
+        return
     st = stream_bench.RxStream("s2x", args.frames, args.seed, learned=True, max_trials=T)
     sc_l = ca.StreamContext(st.ctxs, st.n, ca.STREAM_LEARNED)
     sc_p = ca.StreamContext(st.ctxs, st.n, 0)
@@ -303,7 +318,7 @@ def main():
     achieved = s_l["alg_bytes_decode"] / (s_l["ms_decode"] * 1e-3) / 1e9
 
     model, ncpu, avail = cpu_info()
-    nthr = max(1, min(16, avail))
+    nthr = cpu_threads(avail)
     cpu = None
     parity = None
     if not args.no_cpu:

@@ -163,15 +163,19 @@ class FecContext:
         h = ctypes.c_void_p()
         _check(lib().cusat_fec_ctx_create(ctypes.byref(h), framesize, self.rate, self.modulation,
                                           max_trials, max_batch, flags), "cusat_fec_ctx_create")
+        self._destroy = lib().cusat_fec_ctx_destroy
         self._h = h
         inf = FecInfo()
         _check(lib().cusat_fec_ctx_info(h, ctypes.byref(inf)), "cusat_fec_ctx_info")
         self.info = inf.as_dict()
 
     def close(self):
-        if getattr(self, "_h", None):
-            lib().cusat_fec_ctx_destroy(self._h)
+        h = getattr(self, "_h", None)
+        if h:
             self._h = None
+            # the bound function, not lib(): at interpreter exit the module's
+            # globals may already be gone when the destructor runs
+            self._destroy(h)
 
     __del__ = close
 
@@ -272,12 +276,16 @@ class StreamContext:
         h = ctypes.c_void_p()
         _check(lib().cusat_stream_create(ctypes.byref(h), ctypes.cast(arr, ctypes.c_void_p), len(arr),
                                          max_frames, flags), "cusat_stream_create")
+        self._destroy = lib().cusat_stream_destroy
         self._h = h
 
     def close(self):
-        if getattr(self, "_h", None):
-            lib().cusat_stream_destroy(self._h)
+        h = getattr(self, "_h", None)
+        if h:
             self._h = None
+            # the bound function, not lib(): at interpreter exit the module's
+            # globals may already be gone when the destructor runs
+            self._destroy(h)
 
     __del__ = close
 
@@ -372,12 +380,16 @@ class LearnContext:
         h = ctypes.c_void_p()
         _check(lib().cusat_learn_ctx_create(ctypes.byref(h), fec._h, max_batch, n_iters),
                "cusat_learn_ctx_create")
+        self._destroy = lib().cusat_learn_ctx_destroy
         self._h = h
 
     def close(self):
-        if getattr(self, "_h", None):
-            lib().cusat_learn_ctx_destroy(self._h)
+        h = getattr(self, "_h", None)
+        if h:
             self._h = None
+            # the bound function, not lib(): at interpreter exit the module's
+            # globals may already be gone when the destructor runs
+            self._destroy(h)
 
     __del__ = close
 
