@@ -74,20 +74,31 @@ __device__ __forceinline__ int16_t to_llr(float v)
 // dl = (d - min) g/4, over the points with dl < 16.  LLR = (mn1 - mn0) g/2
 // + 2 (log sum0 - log sum1).
 //
-// Pass 2 skips a point, for the whole wave, when no lane is near it: for
-// the eight points of a group the top NM - 3 label bits are fixed, so a
-// lane's bound is the largest minimum over those bits' values and both
-// values of the three low bits, + 20 / (g/4); a point beyond it has
-// dl >= 20 (1 - a few ulps) > 16 for every bit and adds nothing.  Inside,
+// Pass 1 takes the points eight at a time.  Within a group the top NM - 3
+// label bits are fixed, so those bits' minima take the group's minimum once,
+// and the three low bits take minima of four; a minimum is exact in any
+// order (the distances are never NaN unless the symbol is, and then every
+// minimum keeps its start value either way).
+//
+// Pass 2 skips a point, for the whole wave, when no lane is near it: a
+// lane's bound for a group is the largest minimum over the fixed bits'
+// values and both values of the three low bits, + 20 / (g/4); a point
+// beyond it has dl >= 20 (1 - a few ulps) > 16 for every bit and adds
+// nothing.  COHERENT (the sorted path, where a wave's symbols are
+// neighbours): a bit whose minimum is the lane's overall minimum d0 (the
+// bit's value at the nearest point) has dl = (d - d0) g/4, the same number
+// for all such bits, so its exp is computed once per point; the other bits
+// take their own exp, skipped when no lane of the wave needs one.  Inside,
 // the per-bit update is branch-free: a term left out adds +0.0f, an
 // identity here (the sums start at +0 and every term is > 0).  The values
 // therefore equal the plain two-pass loop over every point.
-template <int NM>
+template <int NM, bool COHERENT>
 __device__ __forceinline__ void demap_apsk_symbol(CUSAT_AS_CONST const float* pts, float re, float im, float gain,
                                                   const uint32_t* col, int N, int j, int16_t* __restrict__ llr)
 {
+    static_assert(NM >= 3, "groups of eight points");
     constexpr int NP = 1 << NM;
-    constexpr int LOW = NM < 3 ? NM : 3;  // label bits that vary within a group of eight
+    constexpr int TOP = NM - 3;  // label bits fixed within a group of eight
     float mn0[NM], mn1[NM], s0[NM], s1[NM];
 #pragma unroll
     for (int b = 0; b < NM; b++) {
@@ -96,25 +107,45 @@ __device__ __forceinline__ void demap_apsk_symbol(CUSAT_AS_CONST const float* pt
         s0[b] = 0.0f;
         s1[b] = 0.0f;
     }
-#pragma unroll 8
-    for (int i = 0; i < NP; i++) {
-        const float dr = re - pts[2 * i], di = im - pts[2 * i + 1];
-        const float d = fmaf(dr, dr, di * di);
+#pragma unroll 2
+    for (int i0 = 0; i0 < NP; i0 += 8) {
+        float d[8];
 #pragma unroll
-        for (int b = 0; b < NM; b++) {
-            if (i & (1 << (NM - 1 - b))) mn1[b] = d < mn1[b] ? d : mn1[b];
-            else                         mn0[b] = d < mn0[b] ? d : mn0[b];
+        for (int k = 0; k < 8; k++) {
+            const float dr = re - pts[2 * (i0 + k)], di = im - pts[2 * (i0 + k) + 1];
+            d[k] = fmaf(dr, dr, di * di);
+        }
+        // label bit 2 (b = NM - 3): points 0-3 / 4-7; bit 1: {0,1,4,5} /
+        // {2,3,6,7}; bit 0 (b = NM - 1): even / odd
+        const float p01 = fminf(d[0], d[1]), p23 = fminf(d[2], d[3]);
+        const float p45 = fminf(d[4], d[5]), p67 = fminf(d[6], d[7]);
+        const float lo2 = fminf(p01, p23), hi2 = fminf(p45, p67);
+        const float lo1 = fminf(p01, p45), hi1 = fminf(p23, p67);
+        const float lo0 = fminf(fminf(d[0], d[2]), fminf(d[4], d[6]));
+        const float hi0 = fminf(fminf(d[1], d[3]), fminf(d[5], d[7]));
+        const float g = fminf(lo2, hi2);
+        mn0[NM - 3] = fminf(mn0[NM - 3], lo2);
+        mn1[NM - 3] = fminf(mn1[NM - 3], hi2);
+        mn0[NM - 2] = fminf(mn0[NM - 2], lo1);
+        mn1[NM - 2] = fminf(mn1[NM - 2], hi1);
+        mn0[NM - 1] = fminf(mn0[NM - 1], lo0);
+        mn1[NM - 1] = fminf(mn1[NM - 1], hi0);
+#pragma unroll
+        for (int b = 0; b < TOP; b++) {
+            if ((i0 >> (NM - 1 - b)) & 1) mn1[b] = fminf(mn1[b], g);
+            else                          mn0[b] = fminf(mn0[b], g);
         }
     }
     const float quarter = gain * 0.25f, half = gain * 0.5f;
     const float margin = 20.0f / quarter;
+    const float d0 = fminf(mn0[NM - 1], mn1[NM - 1]);
     float low_max = 0.0f;
 #pragma unroll
-    for (int b = NM - LOW; b < NM; b++) low_max = fmaxf(low_max, fmaxf(mn0[b], mn1[b]));
+    for (int b = TOP; b < NM; b++) low_max = fmaxf(low_max, fmaxf(mn0[b], mn1[b]));
     for (int i0 = 0; i0 < NP; i0 += 8) {
         float tg = low_max;
 #pragma unroll
-        for (int b = 0; b < NM - LOW; b++) tg = fmaxf(tg, ((i0 >> (NM - 1 - b)) & 1) ? mn1[b] : mn0[b]);
+        for (int b = 0; b < TOP; b++) tg = fmaxf(tg, ((i0 >> (NM - 1 - b)) & 1) ? mn1[b] : mn0[b]);
         tg += margin;
         float px[8], py[8];
 #pragma unroll
@@ -123,11 +154,28 @@ __device__ __forceinline__ void demap_apsk_symbol(CUSAT_AS_CONST const float* pt
             py[k] = pts[2 * (i0 + k) + 1];
         }
 #pragma unroll
-        for (int k = 0; k < (NP < 8 ? NP : 8); k++) {
+        for (int k = 0; k < 8; k++) {
             const int i = i0 + k;
             const float dr = re - px[k], di = im - py[k];
             const float d = fmaf(dr, dr, di * di);
-            if (d < tg) {
+            if (d < tg && COHERENT) {
+                const float dl0 = (d - d0) * quarter;
+                const float e0 = dl0 < 16.0f ? exp_neg(-dl0) : 0.0f;
+#pragma unroll
+                for (int b = 0; b < NM; b++) {
+                    const bool pol = (i >> (NM - 1 - b)) & 1;
+                    const float m = pol ? mn1[b] : mn0[b];
+                    const float dl = (d - m) * quarter;
+                    const bool own = m != d0 && dl < 16.0f;
+                    float e = m == d0 ? e0 : 0.0f;
+                    if (__builtin_amdgcn_ballot_w64(own)) {
+                        const float eb = exp_neg(-dl);
+                        e = own ? eb : e;
+                    }
+                    s1[b] = pol ? s1[b] + e : s1[b];
+                    s0[b] = pol ? s0[b] : s0[b] + e;
+                }
+            } else if (d < tg) {
 #pragma unroll
                 for (int b = 0; b < NM; b++) {
                     const bool pol = (i >> (NM - 1 - b)) & 1;
@@ -148,17 +196,19 @@ __device__ __forceinline__ void demap_apsk_symbol(CUSAT_AS_CONST const float* pt
     }
 }
 
+template <bool COHERENT>
 __device__ __forceinline__ void demap_apsk_any(int n_mod, const float* pts_g, float re, float im, float gain,
                                                const uint32_t* col, int N, int j, int16_t* llr)
 {
     const auto* pts = (CUSAT_AS_CONST const float*)pts_g;
     switch (n_mod) {
-    case 3: demap_apsk_symbol<3>(pts, re, im, gain, col, N, j, llr); break;
-    case 4: demap_apsk_symbol<4>(pts, re, im, gain, col, N, j, llr); break;
-    case 5: demap_apsk_symbol<5>(pts, re, im, gain, col, N, j, llr); break;
-    case 6: demap_apsk_symbol<6>(pts, re, im, gain, col, N, j, llr); break;
-    case 7: demap_apsk_symbol<7>(pts, re, im, gain, col, N, j, llr); break;
-    case 8: demap_apsk_symbol<8>(pts, re, im, gain, col, N, j, llr); break;
+    // the sorted path takes tables of 64 points and more
+    case 3: if constexpr (!COHERENT) demap_apsk_symbol<3, false>(pts, re, im, gain, col, N, j, llr); break;
+    case 4: if constexpr (!COHERENT) demap_apsk_symbol<4, false>(pts, re, im, gain, col, N, j, llr); break;
+    case 5: if constexpr (!COHERENT) demap_apsk_symbol<5, false>(pts, re, im, gain, col, N, j, llr); break;
+    case 6: demap_apsk_symbol<6, COHERENT>(pts, re, im, gain, col, N, j, llr); break;
+    case 7: demap_apsk_symbol<7, COHERENT>(pts, re, im, gain, col, N, j, llr); break;
+    case 8: demap_apsk_symbol<8, COHERENT>(pts, re, im, gain, col, N, j, llr); break;
     default: break;
     }
 }
@@ -184,7 +234,7 @@ __device__ __forceinline__ void demap_symbol(const demap_args& a, int j, const f
         llr[a.col[1] + j] = to_llr(cr * a.psk8_scale);
         llr[a.col[2] + j] = to_llr(ci * a.psk8_scale);
     } else {
-        demap_apsk_any(a.n_mod, a.pts, re, im, a.gain, a.col, a.N, j, llr);
+        demap_apsk_any<false>(a.n_mod, a.pts, re, im, a.gain, a.col, a.N, j, llr);
     }
 }
 
@@ -231,19 +281,27 @@ struct sort_lds {
     int cell[GRID_CELLS];
     uint16_t order[SORTED_DEMAP_MAX_SYMBOLS];
     int wave_sum[SORT_THREADS / 64];
+    int range[2];  // sorted positions of this workgroup's part
 };
 
-// Whole workgroup: one frame of n symbols y (LLRs to llr), constellation pts
-// in global memory, n_mod bits per symbol.
+// Whole workgroup: part `part` of `parts` of one frame of n symbols y (LLRs
+// to llr), constellation pts in global memory, n_mod bits per symbol.  The
+// workgroup sorts the whole frame and demaps the cells that start in its
+// share of the sorted positions.  The order of the symbols inside a cell
+// depends on the LDS atomics' timing, so the parts split at cell starts:
+// which symbols a cell holds, and where it starts, are the same in every
+// workgroup that sorts the frame.
 __device__ __forceinline__ void demap_frame_sorted(sort_lds& L, int n_mod, const float* pts,
                                                    const float2* __restrict__ y, int n, int N, float gain,
-                                                   float scale, const uint32_t* col, int16_t* __restrict__ llr)
+                                                   float scale, const uint32_t* col, int16_t* __restrict__ llr,
+                                                   int part, int parts)
 {
     constexpr int PER = GRID_CELLS / SORT_THREADS;  // cells per thread in the scan
     static_assert(PER * SORT_THREADS == GRID_CELLS, "whole cells per thread");
     const int t = threadIdx.x;
 #pragma unroll
     for (int k = 0; k < PER; k++) L.cell[PER * t + k] = 0;
+    if (t < 2) L.range[t] = n;
     __syncthreads();
     for (int j = t; j < n; j += SORT_THREADS) atomicAdd(&L.cell[grid_cell(y[j], scale)], 1);
     __syncthreads();
@@ -266,9 +324,15 @@ __device__ __forceinline__ void demap_frame_sorted(sort_lds& L, int n_mod, const
     __syncthreads();
     int base = inc - run;
     for (int k = 0; k < w; k++) base += L.wave_sum[k];
+    const int share = (n + parts - 1) / parts;
+    const int want_lo = part * share, want_hi = part == parts - 1 ? n + 1 : (part + 1) * share;
 #pragma unroll
     for (int k = 0; k < PER; k++) {
         L.cell[PER * t + k] = base;
+        // the part runs from the first cell start at or after its share's
+        // start to the first at or after its end (cell 0 starts at 0)
+        if (base >= want_lo) atomicMin(&L.range[0], base);
+        if (base >= want_hi) atomicMin(&L.range[1], base);
         base += v[k];
     }
     __syncthreads();
@@ -277,31 +341,34 @@ __device__ __forceinline__ void demap_frame_sorted(sort_lds& L, int n_mod, const
         L.order[p] = (uint16_t)j;
     }
     __syncthreads();
-    for (int p = t; p < n; p += SORT_THREADS) {
+    const int p_lo = L.range[0], p_hi = L.range[1];
+    for (int p = p_lo + t; p < p_hi; p += SORT_THREADS) {
         const int j = L.order[p];
         const float2 s = y[j];
-        demap_apsk_any(n_mod, pts, s.x, s.y, gain, col, N, j, llr);
+        demap_apsk_any<true>(n_mod, pts, s.x, s.y, gain, col, N, j, llr);
     }
     __syncthreads();  // the LDS is reused by the next frame
 }
 
-// cusat_fec_demap of a constellation of the sorted path: one workgroup per frame
-__global__ void __launch_bounds__(SORT_THREADS) demap_sorted_kernel(demap_args a)
+// cusat_fec_demap of a constellation of the sorted path: `parts` workgroups
+// per frame
+__global__ void __launch_bounds__(SORT_THREADS) demap_sorted_kernel(demap_args a, int parts)
 {
     __shared__ sort_lds L;
-    const int f = blockIdx.x;
+    const int f = blockIdx.x / parts;
     demap_frame_sorted(L, a.n_mod, a.pts, reinterpret_cast<const float2*>(a.sym) + (size_t)f * a.xfec, a.dlen, a.N,
-                       a.gain, a.grid_scale, a.col, a.llr + (size_t)f * a.N);
+                       a.gain, a.grid_scale, a.col, a.llr + (size_t)f * a.N, (int)blockIdx.x - f * parts, parts);
 }
 
 // Mixed-MODCOD batch: a fixed grid takes work items from a queue in device
 // memory (hdr[8], reset by the last workgroup out through hdr[9], so a
-// captured launch replays any plan).  Items 0 .. hdr[3] - 1 are the frames
-// of the sorted path in dlist order, one workgroup each; the next hdr[1]
-// are DEMAP_BLOCK-symbol blocks of the other frames (frame dlist[k].x owns
-// blocks dlist[k].y .. dlist[k + 1].y - 1).  Both parts are in descending
-// cost, so the queue hands out the heavy work first and the tail is cheap.
-// Every symbol gets exactly the arithmetic of demap_kernel.
+// captured launch replays any plan).  Items 0 .. hdr[3] - 1 are the parts
+// of the sorted-path frames in dlist order, one workgroup each; the next
+// hdr[1] are blocks of rx_code::span symbols of the other frames (the
+// entry k of dlist at or after hdr[3] for frame dlist[k].x owns blocks
+// dlist[k].y .. dlist[k + 1].y - 1).  Both are in descending cost, so the queue hands
+// out the heavy work first and the tail is cheap.  Every symbol gets
+// exactly the arithmetic of demap_kernel.
 __global__ void __launch_bounds__(SORT_THREADS)
     demap_mixed_kernel(const rx_code* __restrict__ codes, const rx_rec* __restrict__ recs,
                        const int2* __restrict__ dlist, const rx_scale* __restrict__ snap, int* hdr,
@@ -309,7 +376,7 @@ __global__ void __launch_bounds__(SORT_THREADS)
 {
     __shared__ sort_lds L;
     __shared__ int item_s;
-    const int n = hdr[0], nh = hdr[3];
+    const int nh = hdr[3], n_dl = hdr[4];
     const int total = nh + hdr[1];
     unsigned* qc = reinterpret_cast<unsigned*>(hdr + 8);
     int k_lo = nh;  // a workgroup's block items only grow: their frame only moves forward
@@ -320,7 +387,8 @@ __global__ void __launch_bounds__(SORT_THREADS)
         __syncthreads();
         if (item >= total) break;
         if (item < nh) {
-            const int f = dlist[item].x;
+            const int2 e = dlist[item];
+            const int f = e.x;
             const rx_rec& r = recs[f];
             const rx_code& c = codes[r.code];
             const float gain = (r.flags & RX_SCALE_STATE) ? snap[r.code].gain : r.gain;
@@ -328,11 +396,11 @@ __global__ void __launch_bounds__(SORT_THREADS)
 #pragma unroll
             for (int b = 0; b < 8; b++) col[b] = c.col[b];
             demap_frame_sorted(L, c.n_mod, c.pts, reinterpret_cast<const float2*>(sym) + r.sym_off, c.dlen, c.N,
-                               gain, c.grid_scale, col, llr + r.llr_off);
+                               gain, c.grid_scale, col, llr + r.llr_off, e.y & 0xff, e.y >> 8);
             continue;
         }
         const int blk = item - nh;
-        int lo = k_lo, hi = n - 1;  // last entry with first block <= blk
+        int lo = k_lo, hi = n_dl - 1;  // last entry with first block <= blk
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
             if (dlist[mid].y <= blk) lo = mid;
@@ -361,8 +429,9 @@ __global__ void __launch_bounds__(SORT_THREADS)
             a.psk8_scale = r.psk8_scale;
             a.gain = r.gain;
         }
-        const int j = (blk - e.y) * DEMAP_BLOCK + threadIdx.x;
-        if (j < a.dlen) demap_symbol(a, j, reinterpret_cast<const float2*>(sym) + r.sym_off, llr + r.llr_off);
+        const int j0 = (blk - e.y) * c.span, j1 = min(a.dlen, j0 + c.span);
+        for (int j = j0 + (int)threadIdx.x; j < j1; j += DEMAP_BLOCK)
+            demap_symbol(a, j, reinterpret_cast<const float2*>(sym) + r.sym_off, llr + r.llr_off);
     }
     // the last workgroup to leave resets the queue for the next launch
     if (threadIdx.x == 0) {
@@ -542,7 +611,8 @@ __global__ void __launch_bounds__(64)
 hipError_t launch_demap(const demap_args& a, int batch, hipStream_t stream)
 {
     if (demap_sorted(a.pts != nullptr, a.n_points, a.dlen)) {
-        hipLaunchKernelGGL(demap_sorted_kernel, dim3(batch), dim3(SORT_THREADS), 0, stream, a);
+        const int parts = demap_parts(a.dlen, a.n_points);
+        hipLaunchKernelGGL(demap_sorted_kernel, dim3(batch * parts), dim3(SORT_THREADS), 0, stream, a, parts);
         return hipGetLastError();
     }
     dim3 grid((a.dlen + 255) / 256, batch);

@@ -76,11 +76,30 @@ hipError_t launch_demap(const demap_args& a, int batch, hipStream_t stream);
 // tables of at least SORTED_DEMAP_MIN_POINTS points and at most
 // SORTED_DEMAP_MAX_SYMBOLS demapped symbols.
 constexpr int SORTED_DEMAP_MIN_POINTS = 64;
-constexpr int DEMAP_BLOCK = 512;  // symbols per work item of the other frames of a mixed batch
+constexpr int DEMAP_BLOCK = 512;  // threads of a mixed-batch demap workgroup
 constexpr int SORTED_DEMAP_MAX_SYMBOLS = 16200;
 inline bool demap_sorted(bool table, int n_points, int dlen)
 {
     return table && n_points >= SORTED_DEMAP_MIN_POINTS && dlen <= SORTED_DEMAP_MAX_SYMBOLS;
+}
+// A sorted-path frame is demapped by demap_parts workgroups: each sorts the
+// whole frame and demaps one 64-aligned share of the sorted symbols.
+// Redoing the sort is cheap next to the demap, and a frame then no longer
+// sets the launch's length by itself.
+constexpr int DEMAP_MAX_PARTS = 4;
+constexpr int64_t DEMAP_PART_COST = 1 << 19;  // symbols x points per part
+inline int demap_parts(int dlen, int n_points)
+{
+    const int64_t p = ((int64_t)dlen * n_points + DEMAP_PART_COST / 2) / DEMAP_PART_COST;
+    return p < 1 ? 1 : p > DEMAP_MAX_PARTS ? DEMAP_MAX_PARTS : (int)p;
+}
+// Symbols per work item of a mixed batch's frames outside the sorted path:
+// enough work per item that the queue fetch and the frame lookup stay
+// small beside it (QPSK / 8PSK closed forms cost a few operations per
+// symbol, a table of 2^n_mod points about n_mod 2^n_mod).
+inline int demap_span(bool table, int n_mod)
+{
+    return !table ? 8 * DEMAP_BLOCK : n_mod <= 5 ? 2 * DEMAP_BLOCK : DEMAP_BLOCK;
 }
 // grid scale of a constellation of maximum radius rmax: the grid spans
 // [-1.05 rmax, 1.05 rmax] in both axes
@@ -97,10 +116,11 @@ struct rx_code {
     int has_cap;               // the demapper gain is capped at gain_cap
     float gain_cap;
     float grid_scale;          // demap_grid_scale of the constellation (sorted path)
+    int span;                  // demap_span: symbols per mixed-batch work item
     uint32_t col[8];
 };
 struct rx_rec {
-    int code, blk0;            // blk0: the frame's first 256-symbol demap block in the batch
+    int code, blk0;            // blk0: the frame's first demap block (-1: sorted path)
     int64_t sym_off;           // complex symbols
     int64_t llr_off;           // LLRs
     int64_t hard_off;          // bytes
@@ -121,14 +141,16 @@ struct rx_event {
 struct rx_scale {
     float qpsk_scale, psk8_scale, gain, snr_lin;
 };
-// hdr[0] = frames, hdr[1] = DEMAP_BLOCK-symbol blocks of the frames outside
-// the sorted path, hdr[2] = SNR events, hdr[3] = frames of the sorted path,
-// hdr[8, 9] = the demap launch's work queue and exit count (0 in a fresh
-// plan; the launch leaves them 0) -- device memory.  dlist: per frame
-// (frame number, first block): the sorted-path frames first, costliest
-// first, then the others, costliest first (blocks numbered in that order).
-// One launch: a fixed grid takes work items from the queue -- the
-// sorted-path frames one workgroup each, then the blocks of the others.
+// hdr[0] = frames, hdr[1] = work items (blocks of a code's span of
+// symbols) of the frames outside the sorted path, hdr[2] = SNR events, hdr[3] = work items of the sorted
+// path, hdr[4] = dlist entries, hdr[8, 9] = the demap launch's work queue
+// and exit count (0 in a fresh plan; the launch leaves them 0) -- device
+// memory.  dlist: the sorted-path items first, costliest frame first, each
+// (frame number, part | parts << 8) for the demap_parts of its frame; then
+// per other frame, costliest first, (frame number, first block), blocks
+// numbered in that order.  One launch: a fixed grid takes work items from
+// the queue -- a sorted-path item per workgroup, then the blocks of the
+// others.
 hipError_t launch_demap_mixed(const rx_code* codes, const rx_rec* recs, const int2* dlist, const rx_scale* snap,
                               int* hdr, const float* sym, int16_t* llr, int n_cu, hipStream_t stream);
 // est[e] = decision-directed SNR of the frame of every estimate event

@@ -569,6 +569,7 @@ int upload_codes(cusat_stream_ctx* sc, bool rx_codes)
         o.prbs = c->d_prbs;
         o.dd_pts = c->d_pts;
         o.grid_scale = c->grid_scale;
+        o.span = demap_span(!formula, c->con.bits_per_sym);
         o.has_cap = c->llr_target > 0.0f && c->dmax > 0.0f;
         o.gain_cap = o.has_cap ? 2.0f * c->llr_target / c->dmax : 0.0f;  // as cusat_fec_demap
         o.xfec = c->xfec;
@@ -720,7 +721,7 @@ extern "C" int cusat_stream_create(cusat_stream_ctx** out, cusat_fec_ctx* const*
     sc->off_rx = sc->off_order + al(sizeof(int) * max_frames);
     sc->max_events = 2 * max_frames + n_codes;
     sc->off_dl = sc->off_rx + 256 + al(sizeof(rx_rec) * max_frames);
-    sc->off_ev = sc->off_dl + al(sizeof(int2) * max_frames);
+    sc->off_ev = sc->off_dl + al(sizeof(int2) * max_frames * DEMAP_MAX_PARTS);
     sc->plan_bytes = sc->off_ev + al(sizeof(rx_event) * sc->max_events);
     const size_t bytes = sc->plan_bytes + al(sizeof(code_desc) * n_codes) + al(sizeof(rx_code) * n_codes);
     sc->stage_bytes = sc->plan_bytes + sizeof(code_desc) * n_codes;
@@ -939,7 +940,7 @@ extern "C" int cusat_stream_rx_plan_async(cusat_stream_ctx* sc, const cusat_stre
     // costliest first (arrival order among equals); blocks of the others
     // numbered in that order
     int2* dl = reinterpret_cast<int2*>(img + sc->off_dl);
-    int n_sorted = 0, blk = 0;
+    int n_sorted = 0, n_dl = 0, blk = 0;
     {
         std::vector<int>& ord = sc->demap_order;
         ord.resize(n_frames);
@@ -961,12 +962,16 @@ extern "C" int cusat_stream_rx_plan_async(cusat_stream_ctx* sc, const cusat_stre
         });
         for (int k = 0; k < n_frames; k++) {
             const int f = ord[k];
+            const cusat_fec_ctx* c = sc->codes[frames[f].code];
             if (sorted_path(f)) {
-                dl[k] = make_int2(f, 0);
-                n_sorted++;
+                const int parts = demap_parts(c->dlen, c->con.n_points);
+                for (int p = 0; p < parts; p++) dl[n_dl++] = make_int2(f, p | (parts << 8));
+                n_sorted += parts;
             } else {
-                dl[k] = make_int2(f, blk);
-                blk += (sc->codes[frames[f].code]->dlen + DEMAP_BLOCK - 1) / DEMAP_BLOCK;
+                const bool table = !(c->modulation == CUSAT_MOD_QPSK || c->modulation == CUSAT_MOD_8PSK);
+                const int span = demap_span(table, c->con.bits_per_sym);
+                dl[n_dl++] = make_int2(f, blk);
+                blk += (c->dlen + span - 1) / span;
             }
         }
     }
@@ -992,13 +997,13 @@ extern "C" int cusat_stream_rx_plan_async(cusat_stream_ctx* sc, const cusat_stre
         lo += c->geo.N;
         ho += c->geo.K / 8;
     }
-    for (int k = 0; k < n_frames; k++)
-        if (dl[k].y >= 0 && k >= n_sorted) rec[dl[k].x].blk0 = dl[k].y;
+    for (int k = n_sorted; k < n_dl; k++) rec[dl[k].x].blk0 = dl[k].y;
     memset(hdr, 0, 256);
     hdr[0] = n_frames;
     hdr[1] = blk;
     hdr[2] = n_events;
     hdr[3] = n_sorted;
+    hdr[4] = n_dl;
     rx_event* ev = reinterpret_cast<rx_event*>(img + sc->off_ev);
     for (int e = 0; e < n_events; e++)
         ev[e] = rx_event{events[e].code, events[e].frame, events[e].snr_lin, events[e].snr_db};
@@ -1014,8 +1019,7 @@ extern "C" int cusat_stream_rx_plan_async(cusat_stream_ctx* sc, const cusat_stre
         HIP_TRY(copy_plan(sc, img, n_frames, nj, st, true));
         HIP_TRY(hipMemcpyAsync(d + sc->off_rx, img + sc->off_rx, 256 + n_frames * sizeof(rx_rec),
                                hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemcpyAsync(d + sc->off_dl, img + sc->off_dl, n_frames * sizeof(int2), hipMemcpyHostToDevice,
-                               st));
+        HIP_TRY(hipMemcpyAsync(d + sc->off_dl, img + sc->off_dl, n_dl * sizeof(int2), hipMemcpyHostToDevice, st));
         if (n_events)
             HIP_TRY(hipMemcpyAsync(d + sc->off_ev, img + sc->off_ev, n_events * sizeof(rx_event),
                                    hipMemcpyHostToDevice, st));
