@@ -291,18 +291,39 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
             s2 xp = zero, mp = zero;
             const circ_rec* cc = circ + loff + half * n0;
             if (work) {
+                // both halves with every slot used (exact degree, even CMAX):
+                // the circulant records and APP dwords of all edges first, so
+                // that the loads are in flight together (the other variants
+                // keep them inside the edge guard, which measured faster there)
+                constexpr bool HOIST = EXACT && NE1 == NE0;
+                uint32_t apw[NE0];
+                if (HOIST) {
 #pragma unroll
-                for (int e = 0; e < NE0; e++) {
-                    if (e < ne) {
+                    for (int e = 0; e < NE0; e++) {
                         const circ_rec c = cc[e];
-                        // thread tt's pair dword wraps iff tt < sp; a wrap adds 180
-                        // dwords and swaps the halves
                         const bool wrap = tt < c.sp;
                         const int off = wrap ? tt4w : tt4;
                         sels[e] = (wrap != (c.sel != SEL_ID)) ? SEL_ROT : SEL_ID;
                         ra_[e] = c.app_off + off;
-                        const s2 ap = as_s2(perm(lds_ld(lds_raw, ra_[e]), sels[e]));
+                        apw[e] = lds_ld(lds_raw, ra_[e]);
                         slotm |= (c.slot_off < 0 ? 0u : 1u) << e;
+                    }
+                }
+#pragma unroll
+                for (int e = 0; e < NE0; e++) {
+                    if (e < ne) {
+                        if (!HOIST) {
+                            const circ_rec c = cc[e];
+                            // thread tt's pair dword wraps iff tt < sp; a wrap adds 180
+                            // dwords and swaps the halves
+                            const bool wrap = tt < c.sp;
+                            const int off = wrap ? tt4w : tt4;
+                            sels[e] = (wrap != (c.sel != SEL_ID)) ? SEL_ROT : SEL_ID;
+                            ra_[e] = c.app_off + off;
+                            apw[e] = lds_ld(lds_raw, ra_[e]);
+                            slotm |= (c.slot_off < 0 ? 0u : 1u) << e;
+                        }
+                        const s2 ap = as_s2(perm(apw[e], sels[e]));
                         const s2 o = unpack_msg(rw, e);
                         const s2 x = __builtin_elementwise_sub_sat(ap, o);
                         const s2 m = __builtin_elementwise_max(x, __builtin_elementwise_sub_sat(zero, x));
