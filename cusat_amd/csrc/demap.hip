@@ -142,47 +142,96 @@ __device__ __forceinline__ void demap_apsk_symbol(CUSAT_AS_CONST const float* pt
     float low_max = 0.0f;
 #pragma unroll
     for (int b = TOP; b < NM; b++) low_max = fmaxf(low_max, fmaxf(mn0[b], mn1[b]));
-    for (int i0 = 0; i0 < NP; i0 += 8) {
-        float tg = low_max;
+    if constexpr (COHERENT) {
+        // per (bit, value): a point at d >= min + margin adds nothing to
+        // that bit's sum (dl >= 20 (1 - a few ulps) > 16), for any lane
+        float th0[NM], th1[NM];
 #pragma unroll
-        for (int b = 0; b < TOP; b++) tg = fmaxf(tg, ((i0 >> (NM - 1 - b)) & 1) ? mn1[b] : mn0[b]);
-        tg += margin;
-        float px[8], py[8];
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            px[k] = pts[2 * (i0 + k)];
-            py[k] = pts[2 * (i0 + k) + 1];
+        for (int b = 0; b < NM; b++) {
+            th0[b] = mn0[b] + margin;
+            th1[b] = mn1[b] + margin;
         }
+        for (int i0 = 0; i0 < NP; i0 += 8) {
+            float tg = low_max;
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const int i = i0 + k;
-            const float dr = re - px[k], di = im - py[k];
-            const float d = fmaf(dr, dr, di * di);
-            if (d < tg && COHERENT) {
-                const float dl0 = (d - d0) * quarter;
-                const float e0 = dl0 < 16.0f ? exp_neg(-dl0) : 0.0f;
+            for (int b = 0; b < TOP; b++) tg = fmaxf(tg, ((i0 >> (NM - 1 - b)) & 1) ? mn1[b] : mn0[b]);
+            tg += margin;
+            float px[8], py[8];
 #pragma unroll
-                for (int b = 0; b < NM; b++) {
-                    const bool pol = (i >> (NM - 1 - b)) & 1;
-                    const float m = pol ? mn1[b] : mn0[b];
-                    const float dl = (d - m) * quarter;
-                    const bool own = m != d0 && dl < 16.0f;
-                    float e = m == d0 ? e0 : 0.0f;
-                    if (__builtin_amdgcn_ballot_w64(own)) {
-                        const float eb = exp_neg(-dl);
-                        e = own ? eb : e;
+            for (int k = 0; k < 8; k++) {
+                px[k] = pts[2 * (i0 + k)];
+                py[k] = pts[2 * (i0 + k) + 1];
+            }
+            // the group's fixed bits: their minimum, threshold and running
+            // sum (the sum of that bit value), written back after the group
+            float mt[TOP > 0 ? TOP : 1], tt[TOP > 0 ? TOP : 1], at[TOP > 0 ? TOP : 1];
+#pragma unroll
+            for (int b = 0; b < TOP; b++) {
+                const bool pol = (i0 >> (NM - 1 - b)) & 1;
+                mt[b] = pol ? mn1[b] : mn0[b];
+                tt[b] = pol ? th1[b] : th0[b];
+                at[b] = pol ? s1[b] : s0[b];
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const float dr = re - px[k], di = im - py[k];
+                const float d = fmaf(dr, dr, di * di);
+                if (d < tg) {
+                    const float dl0 = (d - d0) * quarter;
+                    float e0 = 0.0f;
+                    if (__builtin_amdgcn_ballot_w64(dl0 < 16.0f)) e0 = dl0 < 16.0f ? exp_neg(-dl0) : 0.0f;
+#pragma unroll
+                    for (int b = 0; b < NM; b++) {
+                        const bool lpol = (k >> (NM - 1 - b)) & 1;  // low bits: fixed by k
+                        const float m = b < TOP ? mt[b < TOP ? b : 0] : (lpol ? mn1[b] : mn0[b]);
+                        const float thr = b < TOP ? tt[b < TOP ? b : 0] : (lpol ? th1[b] : th0[b]);
+                        if (__builtin_amdgcn_ballot_w64(d < thr)) {
+                            const float dl = (d - m) * quarter;
+                            const bool own = m != d0 && dl < 16.0f;
+                            float e = m == d0 ? e0 : 0.0f;
+                            if (__builtin_amdgcn_ballot_w64(own)) {
+                                const float eb = exp_neg(-dl);
+                                e = own ? eb : e;
+                            }
+                            if (b < TOP)   at[b < TOP ? b : 0] += e;
+                            else if (lpol) s1[b] += e;
+                            else           s0[b] += e;
+                        }
                     }
-                    s1[b] = pol ? s1[b] + e : s1[b];
-                    s0[b] = pol ? s0[b] : s0[b] + e;
                 }
-            } else if (d < tg) {
+            }
 #pragma unroll
-                for (int b = 0; b < NM; b++) {
-                    const bool pol = (i >> (NM - 1 - b)) & 1;
-                    const float dl = (d - (pol ? mn1[b] : mn0[b])) * quarter;
-                    const float e = dl < 16.0f ? exp_neg(-dl) : 0.0f;
-                    s1[b] = pol ? s1[b] + e : s1[b];
-                    s0[b] = pol ? s0[b] : s0[b] + e;
+            for (int b = 0; b < TOP; b++) {
+                if ((i0 >> (NM - 1 - b)) & 1) s1[b] = at[b];
+                else                          s0[b] = at[b];
+            }
+        }
+    } else {
+        for (int i0 = 0; i0 < NP; i0 += 8) {
+            float tg = low_max;
+#pragma unroll
+            for (int b = 0; b < TOP; b++) tg = fmaxf(tg, ((i0 >> (NM - 1 - b)) & 1) ? mn1[b] : mn0[b]);
+            tg += margin;
+            float px[8], py[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                px[k] = pts[2 * (i0 + k)];
+                py[k] = pts[2 * (i0 + k) + 1];
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int i = i0 + k;
+                const float dr = re - px[k], di = im - py[k];
+                const float d = fmaf(dr, dr, di * di);
+                if (d < tg) {
+#pragma unroll
+                    for (int b = 0; b < NM; b++) {
+                        const bool pol = (i >> (NM - 1 - b)) & 1;
+                        const float dl = (d - (pol ? mn1[b] : mn0[b])) * quarter;
+                        const float e = dl < 16.0f ? exp_neg(-dl) : 0.0f;
+                        s1[b] = pol ? s1[b] + e : s1[b];
+                        s0[b] = pol ? s0[b] : s0[b] + e;
+                    }
                 }
             }
         }
