@@ -11,11 +11,12 @@
 // the second lane idle, exactly as the per-code decoder); in learned mode
 // (codes with parameters) a frame whose learned pass does not meet its
 // syndrome is decoded again by the plain decoder in the same workgroup.
-// A second launch checks the BCH syndrome of every frame, one wave per
-// frame (bch_check.hip's algorithm).  In learned mode it also queues the
-// frames that met their syndrome in the learned pass but fail the check,
-// and a third launch (this kernel again, over that queue) decodes them
-// with the plain decoder and checks them again: every frame failing
+// The workgroup checks the BCH syndromes of its decoded frames in rounds,
+// one wave per frame (bch_check.hip's algorithm, each wave with its code's
+// byte table in its own LDS slice), once it holds one frame per wave and
+// when the queue is empty.  In learned mode a frame that met its syndrome
+// in the learned pass but fails the check is decoded again by the plain
+// decoder and checked again in the same round: every frame failing
 // frame_passes_fec_checks (rx_fec_chain.h:193-196) after its learned pass
 // gets the plain decoder, as cusat_rx_fec_learned does.
 // Results equal the per-code launches frame for frame (cusat_ldpc_decode +
@@ -320,26 +321,96 @@ __device__ __attribute__((noinline)) int bch_one(CUSAT_AS_CONST const code_desc_
                          (const uint64_t*)c.bch_pw, c.bch_bytes, c.bch_degree, c.bch_chunk, c.bch_words);
 }
 
-// ctl_off: byte offset in the LDS of the two control ints (queue slot, BCH
-// flag), after the decoders' own LDS.  FIXED: the whole LDS as a static
-// array (address known at compile time, no base add on every LDS access),
-// for streams whose LDS allows one workgroup per CU anyway
+// LDS control area of the stream kernel, at ctl_off (after the decoders'
+// own LDS): the queue slot, bch_one's flag, and the frames this workgroup
+// has decoded whose BCH check is pending (frame, trials, redone, redo flag)
+constexpr int CTL_BYTES = 256;
+constexpr int BCH_SLICE = 256 * 4 * 8;  // a code's byte table (at most 4 words per entry)
+template <int PEND>
+struct stream_ctl {
+    int job, bch_flag;
+    int frame[PEND], res[PEND], fb[PEND], redo[PEND];
+};
+static_assert(sizeof(stream_ctl<12>) <= CTL_BYTES, "control area");
+
+// ctl_off: byte offset in the LDS of the control area.  FIXED: the whole
+// LDS as a static array (address known at compile time, no base add on
+// every LDS access), for streams whose LDS allows one workgroup per CU
+// anyway.
+//
+// A workgroup decodes its frames one after the other and checks their BCH
+// syndromes in rounds, one wave per frame, each wave with its code's byte
+// table in its own LDS slice: when it holds one decoded frame per wave, and
+// when the queue is empty.  A frame of a learned code that met its
+// syndrome in the learned pass but fails the check is decoded again by the
+// plain decoder and checked again (bch_one) in the same round.
 template <bool WIDE, bool FIXED = false>
 __global__ void __launch_bounds__(stream_threads<WIDE>())
     ldpc_stream_kernel(const code_desc* __restrict__ codes, const cusat_stream_frame* __restrict__ frames,
                        const int2* __restrict__ jobs, const int* __restrict__ order,
                        unsigned* counters, const int16_t* __restrict__ llr, uint8_t* __restrict__ hard,
                        int32_t* __restrict__ trials, int32_t* __restrict__ valid,
-                       int32_t* __restrict__ fallback, int32_t* __restrict__ redone, uint4* __restrict__ ws,
-                       size_t ws_v4, int learned, int ctl_off, int redo)
+                       int32_t* __restrict__ fallback, uint4* __restrict__ ws, size_t ws_v4, int learned,
+                       int ctl_off)
 {
+    constexpr int NT = stream_threads<WIDE>();
+    constexpr int PEND = NT / 64;  // one pending frame per wave
     unsigned char* lds_raw = stream_lds<FIXED>();
-    int& job_s = reinterpret_cast<int*>(lds_raw + ctl_off)[0];
+    auto& ctl = *reinterpret_cast<stream_ctl<PEND>*>(lds_raw + ctl_off);
+    const int slices = min(PEND, ctl_off / BCH_SLICE);  // byte tables that fit below the control area
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint4* rec = ws + (size_t)blockIdx.x * ws_v4;
+    int npend = 0;
+    auto bch_round = [&]() {
+        __syncthreads();  // the pending list is written; the decoder's LDS is free
+        for (int g = 0; g < npend; g += slices) {
+            const int k = g + wv;
+            const bool live = wv < slices && k < npend;
+            const int f = live ? __builtin_amdgcn_readfirstlane(ctl.frame[k]) : 0;
+            const int ci = live ? __builtin_amdgcn_readfirstlane(frames[f].code) : 0;
+            const code_desc& c = codes[ci];
+            uint64_t* tab = reinterpret_cast<uint64_t*>(lds_raw + wv * BCH_SLICE);
+            if (live)
+                for (int x = lane; x < 256 * c.bch_words; x += 64) tab[x] = c.bch_lut[x];
+            __syncthreads();
+            if (live) {
+                const uint8_t* h = hard + frames[f].hard_offset;
+                int ok;
+                switch (c.bch_words) {
+                case 1: ok = bch_wave<1>(h, tab, c.bch_pw, c.bch_bytes, c.bch_degree, c.bch_chunk, lane); break;
+                case 2: ok = bch_wave<2>(h, tab, c.bch_pw, c.bch_bytes, c.bch_degree, c.bch_chunk, lane); break;
+                case 3: ok = bch_wave<3>(h, tab, c.bch_pw, c.bch_bytes, c.bch_degree, c.bch_chunk, lane); break;
+                default: ok = bch_wave<4>(h, tab, c.bch_pw, c.bch_bytes, c.bch_degree, c.bch_chunk, lane); break;
+                }
+                if (lane == 0) {
+                    valid[f] = ok;
+                    ctl.redo[k] = learned && c.learned_trials > 0 && !ok && ctl.res[k] >= 0 && !ctl.fb[k];
+                }
+            }
+            __syncthreads();  // the slices are reused; the redo flags are written
+        }
+        for (int k = 0; k < npend; k++) {
+            if (!__builtin_amdgcn_readfirstlane(ctl.redo[k])) continue;
+            const int f = __builtin_amdgcn_readfirstlane(ctl.frame[k]);
+            const int ci = __builtin_amdgcn_readfirstlane(frames[f].code);
+            const code_desc& c = codes[ci];
+            uint8_t* fh = hard + frames[f].hard_offset;
+            const int res = decode_variant<WIDE, FIXED>(codes, ci, c, llr + frames[f].llr_offset, rec, fh, 0);
+            const int ok = bch_one<NT, FIXED>((CUSAT_AS_CONST const code_desc_dev*)codes, ci,
+                                              (CUSAT_AS_GLOBAL const uint8_t*)fh, ctl_off);
+            if (threadIdx.x == 0) {
+                trials[f] = res;
+                valid[f] = ok;
+                if (fallback) fallback[f] = 1;
+            }
+        }
+        __syncthreads();  // the pending list is reused
+        npend = 0;
+    };
     for (;;) {
-        if (threadIdx.x == 0) job_s = (int)atomicAdd(&counters[0], 1u);
+        if (threadIdx.x == 0) ctl.job = (int)atomicAdd(&counters[0], 1u);
         __syncthreads();
-        const int jb = __builtin_amdgcn_readfirstlane(job_s);
+        const int jb = __builtin_amdgcn_readfirstlane(ctl.job);
         __syncthreads();
         if (jb >= (int)counters[2]) break;  // the plan's job count (device: graph replays see new plans)
         const int2 J = jobs[jb];
@@ -349,25 +420,10 @@ __global__ void __launch_bounds__(stream_threads<WIDE>())
             const code_desc& c = codes[ci];
             const int16_t* fl = llr + frames[f].llr_offset;
             uint8_t* fh = hard + frames[f].hard_offset;
-            if (redo) {
-                // a frame whose learned decode met its syndrome with a
-                // codeword the BCH check rejected: the plain decoder and the
-                // BCH check again, in this workgroup
-                const int res = decode_variant<WIDE, FIXED>(codes, ci, c, fl, rec, fh, 0);
-                const int ok = bch_one<stream_threads<WIDE>(), FIXED>((CUSAT_AS_CONST const code_desc_dev*)codes,
-                                                                      ci, (CUSAT_AS_GLOBAL const uint8_t*)fh, ctl_off);
-                if (threadIdx.x == 0) {
-                    trials[f] = res;
-                    valid[f] = ok;
-                    redone[f] = 1;
-                    if (fallback) fallback[f] = 1;
-                }
-                continue;
-            }
-            // learned pass, then the plain decoder at once when the syndrome
-            // is not met; the BCH check follows in its own launch
-            // (a barrier after each decode: its last step packs the hard
-            // decisions from the LDS that the next decode starts writing)
+            // learned pass, then the plain decoder at once when the
+            // syndrome is not met (a barrier after each decode: its last
+            // step packs the hard decisions from the LDS that the next
+            // decode starts writing)
             const bool use_learned = learned && c.learned_trials > 0;
             int res = decode_variant<WIDE, FIXED>(codes, ci, c, fl, rec, fh, use_learned ? 1 : 0);
             __syncthreads();
@@ -378,63 +434,21 @@ __global__ void __launch_bounds__(stream_threads<WIDE>())
             }
             if (threadIdx.x == 0) {
                 trials[f] = res;
-                redone[f] = fb;
                 if (fallback) fallback[f] = fb;
+                ctl.frame[npend] = f;
+                ctl.res[npend] = res;
+                ctl.fb[npend] = fb;
             }
+            if (++npend == PEND) bch_round();
         }
     }
-    // the last workgroup to leave resets the queue for the next launch (and a
-    // redo launch its frame count, which the BCH launch builds up)
+    if (npend) bch_round();
+    // the last workgroup to leave resets the queue for the next launch
     if (threadIdx.x == 0) {
         __threadfence();
         if (atomicAdd(&counters[1], 1u) == gridDim.x - 1) {
             atomicExch(&counters[0], 0u);
             atomicExch(&counters[1], 0u);
-            if (redo) atomicExch(&counters[2], 0u);
-        }
-    }
-}
-
-// BCH check of every planned frame, one wave per frame (the code's byte
-// table staged in the wave's own LDS slice), after the decode launch.  In
-// learned mode a frame of a learned code that met its syndrome in the
-// learned pass but fails the check is queued for the redo launch: job k of
-// the redo list is frame redo_order[k], and redo_ctr[2] counts them.
-constexpr int BCH_WAVES = 4;
-__global__ void __launch_bounds__(64 * BCH_WAVES)
-    stream_bch_kernel(const code_desc* __restrict__ codes, const cusat_stream_frame* __restrict__ frames,
-                      const unsigned* __restrict__ plan_ctr, const uint8_t* __restrict__ hard,
-                      const int32_t* __restrict__ trials, int32_t* __restrict__ valid,
-                      const int32_t* __restrict__ redone, unsigned* redo_ctr, int2* __restrict__ redo_jobs,
-                      int* __restrict__ redo_order, int learned)
-{
-    __shared__ uint64_t tab_all[BCH_WAVES][256 * 4];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int n = (int)plan_ctr[3];
-    const int f = blockIdx.x * BCH_WAVES + w;
-    const bool live = f < n;
-    const int ci = live ? __builtin_amdgcn_readfirstlane(frames[f].code) : 0;
-    const code_desc& c = codes[ci];
-    uint64_t* tab = tab_all[w];
-    const int words = c.bch_words;
-    if (live)
-        for (int k = lane; k < 256 * words; k += 64) tab[k] = c.bch_lut[k];
-    __syncthreads();
-    if (!live) return;  // after the only barrier
-    const uint8_t* h = hard + frames[f].hard_offset;
-    int ok;
-    switch (words) {
-    case 1: ok = bch_wave<1>(h, tab, c.bch_pw, c.bch_bytes, c.bch_degree, c.bch_chunk, lane); break;
-    case 2: ok = bch_wave<2>(h, tab, c.bch_pw, c.bch_bytes, c.bch_degree, c.bch_chunk, lane); break;
-    case 3: ok = bch_wave<3>(h, tab, c.bch_pw, c.bch_bytes, c.bch_degree, c.bch_chunk, lane); break;
-    default: ok = bch_wave<4>(h, tab, c.bch_pw, c.bch_bytes, c.bch_degree, c.bch_chunk, lane); break;
-    }
-    if (lane == 0) {
-        valid[f] = ok;
-        if (learned && c.learned_trials > 0 && !ok && trials[f] >= 0 && !redone[f]) {
-            const int k = (int)atomicAdd(&redo_ctr[2], 1u);
-            redo_order[k] = f;
-            redo_jobs[k] = make_int2(k, 1);
         }
     }
 }
@@ -494,14 +508,6 @@ struct cusat_stream_ctx {
     int64_t llr_max = 0, hard_max = 0;  // per frame
     int16_t* d_llr_work = nullptr;
     uint8_t* d_hard_work = nullptr;
-    // learned mode's redo launch: its counters (256 bytes; [2] = frames
-    // queued by the BCH launch), jobs and order, and the per-frame flag of
-    // the frames the plain decoder redid (all in the plan allocation, after
-    // the code records)
-    unsigned* d_redo_ctr = nullptr;
-    int2* d_redo_jobs = nullptr;
-    int* d_redo_order = nullptr;
-    int32_t* d_redone = nullptr;
     bool rx_planned = false;  // the current plan is an rx plan
     // plan
     int n_frames = 0, n_jobs = 0;
@@ -544,7 +550,6 @@ int stage_plan(const cusat_stream_ctx* sc, const cusat_stream_tuning& tu, unsign
     unsigned* ctr = reinterpret_cast<unsigned*>(img);
     memset(ctr, 0, 256);
     ctr[2] = (unsigned)nj;
-    ctr[3] = (unsigned)n;  // frames, for the BCH launch
     if (n) memcpy(img + 256, sc->frames.data(), n * sizeof(cusat_stream_frame));
     return nj;
 }
@@ -768,7 +773,7 @@ extern "C" int cusat_stream_create(cusat_stream_ctx** out, cusat_fec_ctx* const*
         sc->lds[0] = std::max(sc->lds[0], std::max(lds, 256 * 4 * 8));
         sc->lds[1] = std::max(sc->lds[1], std::max(lds_w, 256 * 4 * 8));
     }
-    for (int w = 0; w < 2; w++) sc->lds[w] += 16;  // control ints
+    for (int w = 0; w < 2; w++) sc->lds[w] += CTL_BYTES;  // control area
     int dev = 0;
     hipGetDevice(&dev);
     hipDeviceProp_t prop;
@@ -798,8 +803,7 @@ extern "C" int cusat_stream_create(cusat_stream_ctx** out, cusat_fec_ctx* const*
     sc->off_dl = sc->off_rx + 256 + al(sizeof(rx_rec) * max_frames);
     sc->off_ev = sc->off_dl + al(sizeof(int2) * max_frames * DEMAP_MAX_PARTS);
     sc->plan_bytes = sc->off_ev + al(sizeof(rx_event) * sc->max_events);
-    const size_t off_redo = sc->plan_bytes + al(sizeof(code_desc) * n_codes) + al(sizeof(rx_code) * n_codes);
-    const size_t bytes = off_redo + 256 + al(sizeof(int2) * max_frames) + 2 * al(sizeof(int) * max_frames);
+    const size_t bytes = sc->plan_bytes + al(sizeof(code_desc) * n_codes) + al(sizeof(rx_code) * n_codes);
     sc->stage_bytes = sc->plan_bytes + sizeof(code_desc) * n_codes;
     for (const cusat_fec_ctx* c : sc->codes) {
         sc->llr_max = std::max<int64_t>(sc->llr_max, c->geo.N);
@@ -831,11 +835,7 @@ extern "C" int cusat_stream_create(cusat_stream_ctx** out, cusat_fec_ctx* const*
     sc->d_order = reinterpret_cast<int*>(base + sc->off_order);
     sc->d_codes = reinterpret_cast<code_desc*>(base + sc->plan_bytes);
     sc->d_rx_codes = reinterpret_cast<rx_code*>(base + sc->plan_bytes + al(sizeof(code_desc) * n_codes));
-    sc->d_redo_ctr = reinterpret_cast<unsigned*>(base + off_redo);
-    sc->d_redo_jobs = reinterpret_cast<int2*>(base + off_redo + 256);
-    sc->d_redo_order = reinterpret_cast<int*>(base + off_redo + 256 + al(sizeof(int2) * max_frames));
-    sc->d_redone = sc->d_redo_order + al(sizeof(int) * max_frames) / sizeof(int);
-    if (hipMemset(sc->d_counters, 0, 256) != hipSuccess || hipMemset(sc->d_redo_ctr, 0, 256) != hipSuccess ||
+    if (hipMemset(sc->d_counters, 0, 256) != hipSuccess ||
         hipMemset(sc->d_snap, 0, n_codes * sizeof(rx_scale)) != hipSuccess) {
         cusat_stream_destroy(sc);
         set_last_error("counter init failed");
@@ -945,19 +945,9 @@ extern "C" int cusat_stream_get_tuning(const cusat_stream_ctx* sc, cusat_stream_
 
 namespace {
 
-template <bool WIDE, bool FIXED>
-void launch_stream_kernel(cusat_stream_ctx* sc, int grid, int lds, hipStream_t st, const int2* jobs, const int* order,
-                          unsigned* counters, const int16_t* llr, uint8_t* hard, int32_t* trials, int32_t* valid,
-                          int32_t* fallback, int learned, int ctl_off, int redo)
-{
-    hipLaunchKernelGGL((ldpc_stream_kernel<WIDE, FIXED>), dim3(grid), dim3(stream_threads<WIDE>()), lds, st,
-                       sc->d_codes, sc->d_frames, jobs, order, counters, llr, hard, trials, valid, fallback,
-                       sc->d_redone, sc->d_ws, sc->ws_v4, learned, ctl_off, redo);
-}
-
-// The decode launch (persistent LDPC, learned pass and plain decoder on a
-// syndrome failure), the BCH launch, and in learned mode the redo launch
-// for the frames whose learned codeword the BCH check rejected.
+// The decode launch: persistent LDPC (learned pass and plain decoder on a
+// syndrome failure), the BCH checks in rounds, the plain redo of learned
+// codewords the check rejects.
 hipError_t launch_stream(cusat_stream_ctx* sc, const int16_t* llr, uint8_t* hard, int32_t* trials, int32_t* valid,
                          int32_t* fallback, void* stream)
 {
@@ -968,23 +958,18 @@ hipError_t launch_stream(cusat_stream_ctx* sc, const int16_t* llr, uint8_t* hard
     const int w = sc->tuning.split_edges;
     const int learned = (sc->flags & CUSAT_STREAM_LEARNED) ? 1 : 0;
     hipStream_t st = (hipStream_t)stream;
-    auto run = [&](const int2* jobs, const int* order, unsigned* ctr, int redo) {
-        if (w && sc->lds[1] > ldpc_dev::SPLIT_LDS_STATIC / 2)  // as fit_kernel chose
-            launch_stream_kernel<true, true>(sc, grid, 0, st, jobs, order, ctr, llr, hard, trials, valid, fallback,
-                                             learned, sc->lds[1] - 16, redo);
-        else if (w)
-            launch_stream_kernel<true, false>(sc, grid, sc->lds[1], st, jobs, order, ctr, llr, hard, trials, valid,
-                                              fallback, learned, sc->lds[1] - 16, redo);
-        else
-            launch_stream_kernel<false, false>(sc, grid, sc->lds[0], st, jobs, order, ctr, llr, hard, trials, valid,
-                                               fallback, learned, sc->lds[0] - 16, redo);
-    };
-    run(sc->d_jobs, sc->d_order, sc->d_counters, 0);
-    const int bgrid = (std::max(1, sc->max_frames) + BCH_WAVES - 1) / BCH_WAVES;
-    hipLaunchKernelGGL(stream_bch_kernel, dim3(bgrid), dim3(64 * BCH_WAVES), 0, st, sc->d_codes, sc->d_frames,
-                       sc->d_counters, hard, trials, valid, sc->d_redone, sc->d_redo_ctr, sc->d_redo_jobs,
-                       sc->d_redo_order, learned);
-    if (learned) run(sc->d_redo_jobs, sc->d_redo_order, sc->d_redo_ctr, 1);
+    if (w && sc->lds[1] > ldpc_dev::SPLIT_LDS_STATIC / 2)  // as fit_kernel chose
+        hipLaunchKernelGGL((ldpc_stream_kernel<true, true>), dim3(grid), dim3(stream_threads<true>()), 0, st,
+                           sc->d_codes, sc->d_frames, sc->d_jobs, sc->d_order, sc->d_counters, llr, hard, trials,
+                           valid, fallback, sc->d_ws, sc->ws_v4, learned, sc->lds[1] - CTL_BYTES);
+    else if (w)
+        hipLaunchKernelGGL(ldpc_stream_kernel<true>, dim3(grid), dim3(stream_threads<true>()), sc->lds[1], st,
+                           sc->d_codes, sc->d_frames, sc->d_jobs, sc->d_order, sc->d_counters, llr, hard, trials,
+                           valid, fallback, sc->d_ws, sc->ws_v4, learned, sc->lds[1] - CTL_BYTES);
+    else
+        hipLaunchKernelGGL(ldpc_stream_kernel<false>, dim3(grid), dim3(stream_threads<false>()), sc->lds[0], st,
+                           sc->d_codes, sc->d_frames, sc->d_jobs, sc->d_order, sc->d_counters, llr, hard, trials,
+                           valid, fallback, sc->d_ws, sc->ws_v4, learned, sc->lds[0] - CTL_BYTES);
     return hipGetLastError();
 }
 
