@@ -64,9 +64,10 @@ bool ldpc_split_available(int cmax, bool exact, int lanes)
 
 int ldpc_lds_bytes_split(int K, int R, int n_slots, int n_steps, int n_groups, int lanes)
 {
-    // + the partial exchange area, + 6 ints per step: the schedule entries
-    // are (layer, lay_off, count, 0) per lane instead of a layer id
-    return ldpc_lds_bytes(K, R, n_slots, n_steps, n_groups) + 3 * lanes * 2 * 180 * 4 + 6 * n_steps * 4;
+    // + the partial exchange area, + 10 ints per step: the schedule entries
+    // are (layer, lay_off, count, group) per lane instead of a layer id,
+    // and each lane's learned (beta, alpha)
+    return ldpc_lds_bytes(K, R, n_slots, n_steps, n_groups) + 3 * lanes * 2 * 180 * 4 + 10 * n_steps * 4;
 }
 
 size_t ldpc_msg_bytes_split(int q, int cmax)

@@ -172,6 +172,99 @@ __device__ __forceinline__ int wg_any(int v, int* flags, int& k)
 // VN group's first pair dword, number of slots, the slots' dword indices.
 constexpr int GROUP_WORDS = 8;
 
+// Channel LLRs of one frame into the LDS APP layout with the int16-APP input
+// clamp -32768 -> -32767 (ldpc_decode_kernel.cu:655-670): info VN v at
+// app16[vn_index(v)], parity c = q j + i at dword i * 180 + j mod 180 (low
+// half for j < 180).  Whole workgroup (NTHR threads), no barrier.
+//
+// With an 8-byte-aligned llr (every frame of a packed batch or stream), the
+// loads are 8 bytes wide and issued eight at a time before their first use:
+// the frame's LLRs then arrive in a few memory round trips instead of one
+// round trip per element.  An info unit is 4 LLRs of each half of a VN
+// group (x0 .. x0 + 3 and x0 + 180 .. x0 + 183), which are exactly 4 pair
+// dwords: one 16-byte LDS write.  A parity unit is 4 consecutive LLRs c;
+// (i, j) comes from one division per unit and an increment per element.
+template <int NTHR>
+__device__ __forceinline__ void load_frame_llr(uint32_t* app32, int16_t* par16, const int16_t* __restrict__ llr,
+                                               int K, int R, int q, int t)
+{
+    int16_t* app16 = reinterpret_cast<int16_t*>(app32);
+    if ((reinterpret_cast<uintptr_t>(llr) & 7) != 0) {
+        for (int v = t; v < K; v += NTHR) app16[vn_index(v)] = (int16_t)max((int)llr[v], -32767);
+        for (int c = t; c < R; c += NTHR) {
+            const int i = c % q, j = c / q;
+            par16[i * CIRC + (j < HALF ? 2 * j : 2 * (j - HALF) + 1)] = (int16_t)max((int)llr[K + c], -32767);
+        }
+        return;
+    }
+    const s2 lo = splat(-32767);
+    const uint2* src = reinterpret_cast<const uint2*>(llr);  // 4 LLRs each
+    constexpr int UI = 4;  // info units per batch (2 loads each)
+    const int nu = K / 8;  // K is a multiple of 360: 45 units per VN group
+    for (int u0 = t; u0 < nu; u0 += UI * NTHR) {
+        uint2 a[UI], b[UI];
+#pragma unroll
+        for (int k = 0; k < UI; k++) {
+            const int u = u0 + k * NTHR;
+            const int r = u / 45, e = 90 * r + (u - 45 * r);  // uint2 index of LLR 360 r + x0
+            a[k] = make_uint2(0, 0);
+            b[k] = make_uint2(0, 0);
+            if (u < nu) {
+                a[k] = src[e];
+                b[k] = src[e + 45];  // + 180 LLRs
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < UI; k++) {
+            const int u = u0 + k * NTHR;
+            if (u < nu) {
+                const int r = u / 45, x0 = 4 * (u - 45 * r);
+                // pair dword x = (LLR 360 r + x, LLR 360 r + 180 + x)
+                uint4 w;
+                w.x = bits(__builtin_elementwise_max(as_s2(__builtin_amdgcn_perm(b[k].x, a[k].x, 0x05040100u)), lo));
+                w.y = bits(__builtin_elementwise_max(as_s2(__builtin_amdgcn_perm(b[k].x, a[k].x, 0x07060302u)), lo));
+                w.z = bits(__builtin_elementwise_max(as_s2(__builtin_amdgcn_perm(b[k].y, a[k].y, 0x05040100u)), lo));
+                w.w = bits(__builtin_elementwise_max(as_s2(__builtin_amdgcn_perm(b[k].y, a[k].y, 0x07060302u)), lo));
+                *reinterpret_cast<uint4*>(app32 + 180 * r + x0) = w;
+            }
+        }
+    }
+    constexpr int UP = 8;  // parity units per batch
+    const int np = R / 4;
+    const uint2* psrc = src + K / 4;
+    const float inv_q = 1.0f / (float)q;
+    for (int u0 = t; u0 < np; u0 += UP * NTHR) {
+        uint2 a[UP];
+#pragma unroll
+        for (int k = 0; k < UP; k++) {
+            const int u = u0 + k * NTHR;
+            a[k] = u < np ? psrc[u] : make_uint2(0, 0);
+        }
+#pragma unroll
+        for (int k = 0; k < UP; k++) {
+            const int u = u0 + k * NTHR;
+            if (u < np) {
+                const int c0 = 4 * u;
+                // j = c0 / q: the float quotient is within one of it (c0 < 2^17)
+                int j = (int)((float)c0 * inv_q);
+                if (j * q > c0) j--;
+                if ((j + 1) * q <= c0) j++;
+                int i = c0 - j * q;
+                const uint32_t w[2] = {a[k].x, a[k].y};
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const int v = (int)(int16_t)(w[e >> 1] >> (16 * (e & 1)));
+                    par16[i * CIRC + (j < HALF ? 2 * j : 2 * (j - HALF) + 1)] = (int16_t)max(v, -32767);
+                    if (++i == q) {
+                        i = 0;
+                        j++;
+                    }
+                }
+            }
+        }
+    }
+}
+
 // Iteration 0's old messages are zero (ldpc_decode_kernel.cu:495-581).  Its
 // record prefetches read this never-written, zero-initialised array instead
 // of the frame's records: the load stays unconditional (a pointer select,
@@ -226,13 +319,7 @@ __device__ __forceinline__ int decode_frame(unsigned char* lds_raw, const int16_
     if (t < 3) flg[t] = 0;
     int any_k = 0;
 
-    // Load with the int16-APP input clamp (ldpc_decode_kernel.cu:655-670):
-    // -32768 -> -32767.
-    for (int v = t; v < p.K; v += NTHR) app16[vn_index(v)] = (int16_t)max((int)llr[v], -32767);
-    for (int c = t; c < p.R; c += NTHR) {
-        const int i = c % q, j = c / q;
-        par16[i * CIRC + (j < HALF ? 2 * j : 2 * (j - HALF) + 1)] = (int16_t)max((int)llr[p.K + c], -32767);
-    }
+    load_frame_llr<NTHR>(app32, par16, llr, p.K, p.R, q, t);
     __syncthreads();
 
     // Message records are not cleared: iteration 0 reads zero_records

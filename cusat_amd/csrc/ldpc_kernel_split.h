@@ -85,9 +85,11 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
     // count, and the step's first collision group, so that a step's first
     // global access is its circulant records
     int4* sched_l = reinterpret_cast<int4*>(xch + 3 * LANES * 2 * HALF);
-    int* grp_l = reinterpret_cast<int*>(sched_l + 2 * p.n_steps);  // [n_groups][8], 16-byte aligned
-    int* goff_l = grp_l + GROUP_WORDS * p.n_groups;
-    int* flg = goff_l + p.n_steps + 1;  // [3] wg_any flags
+    // per (step, lane): the learned (beta, alpha) of the current iteration,
+    // written during the iteration's syndrome pass
+    int2* parl = reinterpret_cast<int2*>(sched_l + 2 * p.n_steps);
+    int* grp_l = reinterpret_cast<int*>(parl + 2 * p.n_steps);  // [n_groups][8], 16-byte aligned
+    int* flg = grp_l + GROUP_WORDS * p.n_groups;  // [3] wg_any flags
     int16_t* app16 = reinterpret_cast<int16_t*>(lds_raw);
     int16_t* par16 = reinterpret_cast<int16_t*>(par32);
 
@@ -113,17 +115,11 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
         sched_l[k] = make_int4(i, lay_off[ic], EXACT ? CMAX : lay_cnt[ic],
                                grp_off[k >> 1] | (step_defer[k >> 1] ? 1 << 30 : 0));
     }
-    for (int k = t; k <= p.n_steps; k += NTHR) goff_l[k] = grp_off[k];
     for (int k = t; k < GROUP_WORDS * p.n_groups; k += NTHR) grp_l[k] = groups[k];
     if (t < 3) flg[t] = 0;
     int any_k = 0;
     const int n_groups_all = p.n_groups;  // first group past the last step
-    // input clamp of the int16 APP (ldpc_decode_kernel.cu:655-670): -32768 -> -32767
-    for (int v = t; v < p.K; v += NTHR) app16[vn_index(v)] = (int16_t)max((int)llr[v], -32767);
-    for (int c = t; c < p.R; c += NTHR) {
-        const int i = c % q, j = c / q;
-        par16[i * CIRC + (j < HALF ? 2 * j : 2 * (j - HALF) + 1)] = (int16_t)max((int)llr[p.K + c], -32767);
-    }
+    load_frame_llr<NTHR>(app32, par16, llr, p.K, p.R, q, t);
     __syncthreads();
 
     // Message records are not cleared: iteration 0 reads zero_records
@@ -131,27 +127,16 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
     // layer once per iteration).
     const int tl = act ? tt : HALF - 1;
     auto rec_at = [&](int i, int pair) { return rec_base + (((size_t)i * 2 + half) * HALF + pair) * R::V4; };
-    // Learned parameters of a (iteration, layer): vector buffer loads issued
-    // one step ahead with the message-record prefetch (a scalar load after
-    // the step's barrier would stall every wave of the workgroup); loads
-    // past the table (iterations >= n_param_iters are plain) are not made.
-    const int n_par = p.n_param_iters * q;
-    const __amdgpu_buffer_rsrc_t rs_beta =
-        __builtin_amdgcn_make_buffer_rsrc((void*)beta, (short)0, beta ? n_par * 4 : 0, BUF_RSRC_WORD3);
-    const __amdgpu_buffer_rsrc_t rs_alpha =
-        __builtin_amdgcn_make_buffer_rsrc((void*)alpha, (short)0, alpha ? n_par * 4 : 0, BUF_RSRC_WORD3);
-    auto par_load = [&](int itx, int ix, int& b, int& a) {
-        b = 1;
-        a = 0;
-        if (itx < p.n_param_iters) {
-            if (beta) b = __builtin_amdgcn_raw_buffer_load_b32(rs_beta, (itx * q + ix) * 4, 0, 0);
-            if (alpha) a = __builtin_amdgcn_raw_buffer_load_b32(rs_alpha, (itx * q + ix) * 4, 0, 0);
-        }
-    };
+    // Learned parameters: the (beta, alpha) of every (step, lane) entry of an
+    // iteration are loaded by thread 2 s + lane during that iteration's
+    // syndrome pass and stored in parl before the pass's barrier; a step
+    // reads its own entry at the exchange.  Plain iterations (>= n_param_iters)
+    // load nothing.  (Loaded one step ahead in registers instead, the value
+    // was copied at the loop's join points, and each copy waited for every
+    // load in flight -- the next step's record prefetch included.)
     static_assert(R::V4 <= 4, "a layer's records fit zero_records");
     uint4 ra[R::V4], rb[R::V4];
-    int pb[2], pa[2];  // beta, alpha of the step whose record is in ra / rb
-    int si[2], so[2], sc[2], sg0[2];  // its layer, lay_off, circulant count, first group
+    int si[2], so[2], sc[2], sg0[2];  // layer, lay_off, circulant count, first group of the step in ra / rb
     {
         const int4 e0 = sched_l[lane];
         si[0] = __builtin_amdgcn_readfirstlane(e0.x);
@@ -162,26 +147,35 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
         so[1] = 0;
         sc[1] = 0;
         sg0[1] = 0;
-        const int i0 = max(si[0], 0);
 #pragma unroll
         for (int k = 0; k < R::V4; k++) {
             ra[k] = make_uint4(0, 0, 0, 0);  // step 0 of iteration 0
             rb[k] = make_uint4(0, 0, 0, 0);
         }
-        par_load(0, i0, pb[0], pa[0]);
-        pb[1] = 1;
-        pa[1] = 0;
     }
 
     int res = -1;
     for (int it = 0;; it++) {
-        // ---- syndrome (half 0 evaluates whole check pairs), early exit ----
+        const bool lrn = it < p.n_param_iters;  // wave-uniform
+        int pbv = 1, pav = 0;
+        if (lrn && t < 2 * p.n_steps) {
+            const int i = sched_l[t].x;
+            if (i >= 0) {
+                if (beta) pbv = beta[it * q + i];
+                if (alpha) pav = alpha[it * q + i];
+            }
+        }
+        // ---- syndrome of the APP signs, early exit (ldpc_decode_kernel.cu:689-749) ----
+        // Per barrier, two schedule steps: wave group (lane, half) evaluates
+        // the whole check pairs of its lane's layer in step s + half.  The
+        // result is the same for any order of the checks.
         int ok = 1;
-        for (int s = 0; s < p.n_steps; s++) {
-            const int4 se = sched_l[2 * s + lane];
+        for (int s = 0; s < p.n_steps; s += 2) {
+            const int sx = min(s + half, p.n_steps - 1);
+            const int4 se = sched_l[2 * sx + lane];
             const int i = __builtin_amdgcn_readfirstlane(se.x);
             int bad = 0;
-            if (act && i >= 0 && half == 0) {
+            if (act && i >= 0 && s + half < p.n_steps) {
                 uint32_t sg = par32[i * HALF + tt];
                 if (i > 0) {
                     sg ^= par32[(i - 1) * HALF + tt];
@@ -205,6 +199,13 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
             }
             if (wg_any(bad, flg, any_k)) { ok = 0; break; }
         }
+        // the learned parameters, loaded before the scan: read at the
+        // exchanges, after the first step's barrier
+        if (lrn && t < 2 * p.n_steps) parl[t] = make_int2(pbv, pav);
+        // vmcnt(0) in uniform control flow: the compiler cannot tie the
+        // parameter loads to the store's branch and would otherwise count
+        // them as outstanding in the step loop, waiting at every step
+        __builtin_amdgcn_s_waitcnt(0x0F70);
         if (ok) { res = p.max_trials - it; break; }
         if (it == p.max_trials) break;
 
@@ -244,7 +245,6 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                 const uint4* src = lay + tl * R::V4;
 #pragma unroll
                 for (int k = 0; k < R::V4; k++) nxt[k] = src[k];
-                par_load(itn, inx, pb[hs ^ 1], pa[hs ^ 1]);
             }
             const bool work = act && i >= 0;
             const s2 zero = splat(0);
@@ -371,8 +371,12 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                     mn1 = __builtin_elementwise_min(mn1, om1);
                     sg ^= xsg[po];
                 }
-                const int bt = __builtin_amdgcn_readfirstlane(pb[hs]);
-                const int al = __builtin_amdgcn_readfirstlane(pa[hs]);
+                int bt = 1, al = 0;
+                if (lrn) {
+                    const int2 pr = parl[2 * s + lane];
+                    bt = __builtin_amdgcn_readfirstlane(pr.x);
+                    al = __builtin_amdgcn_readfirstlane(pr.y);
+                }
                 const u2 beta2 = as_u2(splat(bt));
                 // offset once per check: max(min|x| - beta, 0)
                 const s2 b1 = as_s2u(__builtin_elementwise_sub_sat(as_u2(mn1), beta2));
