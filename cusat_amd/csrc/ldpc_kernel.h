@@ -265,6 +265,31 @@ __device__ __forceinline__ void load_frame_llr(uint32_t* app32, int16_t* par16, 
     }
 }
 
+// Hard decisions of the K info VNs, MSB first (ldpc_decode_kernel.cu:468-493):
+// byte B holds the APP signs of VNs 8B .. 8B + 7.  Within VN group r (45
+// bytes, pair dwords 180 r ..): byte j < 22 is the low halves of dwords
+// 8j .. 8j + 7, byte j > 22 the high halves of dwords 8j - 180 .. 8j - 173,
+// byte 22 the low halves of dwords 176 .. 179 and the high halves of 0 .. 3.
+// Whole workgroup, no barrier.
+template <int NTHR>
+__device__ __forceinline__ void store_hard(const uint32_t* app32, uint8_t* __restrict__ h, int K, int t)
+{
+    for (int b = t; b < (K >> 3); b += NTHR) {
+        const int r = b / 45, j = b - 45 * r;
+        const uint32_t* g = app32 + 180 * r;
+        const int d0 = j < 22 ? 8 * j : (j > 22 ? 8 * j - 180 : 176);
+        const uint4 w0 = *reinterpret_cast<const uint4*>(g + d0);
+        uint4 w1 = *reinterpret_cast<const uint4*>(g + d0 + 4);  // in the LDS for j = 22 too, unused there
+        if (j == 22) w1 = *reinterpret_cast<const uint4*>(g);
+        const int sa = j <= 22 ? 15 : 31;  // sign bit of the first four VNs' half
+        const int sb = j < 22 ? 15 : 31;   // and of the last four
+        const uint32_t byte = ((w0.x >> sa) & 1u) << 7 | ((w0.y >> sa) & 1u) << 6 | ((w0.z >> sa) & 1u) << 5 |
+                              ((w0.w >> sa) & 1u) << 4 | ((w1.x >> sb) & 1u) << 3 | ((w1.y >> sb) & 1u) << 2 |
+                              ((w1.z >> sb) & 1u) << 1 | ((w1.w >> sb) & 1u);
+        h[b] = (uint8_t)byte;
+    }
+}
+
 // Iteration 0's old messages are zero (ldpc_decode_kernel.cu:495-581).  Its
 // record prefetches read this never-written, zero-initialised array instead
 // of the frame's records: the load stays unconditional (a pointer select,
@@ -302,7 +327,6 @@ __device__ __forceinline__ int decode_frame(unsigned char* lds_raw, const int16_
     int* goff_l = sched_l + 2 * p.n_steps;                            // [n_steps + 1]
     int* grp_l = goff_l + p.n_steps + 1;                              // [n_groups][8]
     int* flg = grp_l + GROUP_WORDS * p.n_groups;                      // [3] wg_any flags
-    int16_t* app16 = reinterpret_cast<int16_t*>(lds_raw);
     int16_t* par16 = reinterpret_cast<int16_t*>(par32);
 
     constexpr int NTHR = LANES * LANE_THREADS;
@@ -573,12 +597,7 @@ __device__ __forceinline__ int decode_frame(unsigned char* lds_raw, const int16_
     }
 
     // hard decisions, K bits MSB first (ldpc_decode_kernel.cu:468-493)
-    for (int b = t; b < (p.K >> 3); b += NTHR) {
-        int byte = 0;
-#pragma unroll
-        for (int k = 0; k < 8; k++) byte |= (app16[vn_index(8 * b + k)] < 0 ? 1 : 0) << (7 - k);
-        h[b] = (uint8_t)byte;
-    }
+    store_hard<NTHR>(app32, h, p.K, t);
     return res;
 }
 

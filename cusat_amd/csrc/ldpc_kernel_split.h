@@ -90,7 +90,6 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
     int2* parl = reinterpret_cast<int2*>(sched_l + 2 * p.n_steps);
     int* grp_l = reinterpret_cast<int*>(parl + 2 * p.n_steps);  // [n_groups][8], 16-byte aligned
     int* flg = grp_l + GROUP_WORDS * p.n_groups;  // [3] wg_any flags
-    int16_t* app16 = reinterpret_cast<int16_t*>(lds_raw);
     int16_t* par16 = reinterpret_cast<int16_t*>(par32);
 
     constexpr int NTHR = LANES * SPLIT_LANE_THREADS;
@@ -482,12 +481,7 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
         }
     }
 
-    for (int b = t; b < (p.K >> 3); b += NTHR) {
-        int byte = 0;
-#pragma unroll
-        for (int k = 0; k < 8; k++) byte |= (app16[vn_index(8 * b + k)] < 0 ? 1 : 0) << (7 - k);
-        h[b] = (uint8_t)byte;
-    }
+    store_hard<NTHR>(app32, h, p.K, t);
     return res;
 }
 
