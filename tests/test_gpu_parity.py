@@ -207,3 +207,33 @@ def test_every_code_against_oracle(fs, rate, es):
     """All 51 LDPC codes (34 normal, 17 short) of the tables: demap, decode
     and BCH bit-exact against the oracle on two frames each."""
     _chain_parity(fs, rate, "QPSK", es, 2, 31)
+
+
+@pytest.mark.parametrize("offset", [1, 2, 3])
+@pytest.mark.parametrize("fs,rate,mod,es", [(0, "3/4", "8PSK", 7.6), (1, "1/2", "QPSK", 2.0)])
+def test_unaligned_llr_input(fs, rate, mod, es, offset):
+    """LLRs at an address that is not 8-byte aligned (a view `offset` int16
+    elements into a buffer) take the decoder's element-wise load path: the
+    same hard bits and trials as the aligned input and as the oracle."""
+    import torch
+    B, T = 6, 50
+    ctx = ca.FecContext(fs, rate, mod, max_trials=T, max_batch=B)
+    sym, _, _ = ca.gen_frames(fs, rate, mod, 91 + offset, B, es)
+    nv, gain = snr_scales(es)
+    llr = oracle_demap(ctx, sym, nv, gain)
+    N = llr.shape[1]
+    print("case", fs, rate, mod, es, "offset", offset, flush=True)
+    buf = torch.zeros(B * N + 8, dtype=torch.int16, device="cuda")
+    buf[offset:offset + B * N] = torch.from_numpy(llr.reshape(-1)).cuda()
+    view = buf[offset:offset + B * N]
+    assert view.data_ptr() % 8 != 0
+    hard = torch.zeros((B, ctx.info["k_ldpc"] // 8), dtype=torch.uint8, device="cuda")
+    tr = torch.full((B,), -7, dtype=torch.int32, device="cuda")
+    ctx.decode(view, hard, tr, B)
+    torch.cuda.synchronize()
+    hard_a, tr_a = gpu_decode(ctx, llr)
+    hard_o, tr_o = oracle_code(fs, rate).decode(llr, T)
+    np.testing.assert_array_equal(tr_a, tr_o)
+    np.testing.assert_array_equal(hard_a, hard_o)
+    np.testing.assert_array_equal(tr.cpu().numpy(), tr_o)
+    np.testing.assert_array_equal(hard.cpu().numpy(), hard_o)
