@@ -154,9 +154,11 @@ class GpuFrames:
 
 def train(framesize=0, rate="2/3", modulation="16APSK", n_iters=20, batch=512, steps=400,
           es_lo=8.9, es_hi=9.6, pool_frames=2048, seed=20240611, kappa=0.5, lr_alpha=0.005,
-          lr_beta=0.02, quantized=True, log=print, frames="host"):
+          lr_beta=0.02, quantized=True, log=print, frames="host", alpha0=1.0, beta0=1.0):
     """frames: "host" = a pool of host-generated codewords re-noised per
-    batch; "gpu" = fresh codewords per batch from the GPU generator."""
+    batch; "gpu" = fresh codewords per batch from the GPU generator.
+    alpha0, beta0: the starting scale and offset of every (iteration,
+    layer) (1, 1 = the plain decoder)."""
     import torch
     import cusat_amd as ca
     ctx = ca.FecContext(framesize, rate, modulation, max_trials=50, max_batch=max(batch, 64))
@@ -167,8 +169,8 @@ def train(framesize=0, rate="2/3", modulation="16APSK", n_iters=20, batch=512, s
         pool = FramePool(ctx, framesize, rate, modulation, pool_frames, seed)
     lc = ca.LearnContext(ctx, batch, n_iters)
     dev = torch.device("cuda")
-    alpha = torch.ones((n_iters, q), dtype=torch.float32, device=dev)
-    beta = torch.ones((n_iters, q), dtype=torch.float32, device=dev)
+    alpha = torch.full((n_iters, q), float(alpha0), dtype=torch.float32, device=dev)
+    beta = torch.full((n_iters, q), float(beta0), dtype=torch.float32, device=dev)
     ma, va, mb, vb = [torch.zeros_like(alpha) for _ in range(4)]
     ga, gb = torch.zeros_like(alpha), torch.zeros_like(beta)
     loss = torch.zeros(n_iters, dtype=torch.float32, device=dev)
@@ -210,6 +212,7 @@ def train(framesize=0, rate="2/3", modulation="16APSK", n_iters=20, batch=512, s
                          "pool_frames": pool_frames if frames == "host" else None, "frames": frames,
                          "seed": seed, "kappa": kappa,
                          "lr_alpha": lr_alpha, "lr_beta": lr_beta, "quantized": quantized,
+                         "alpha0": alpha0, "beta0": beta0,
                          "history": hist}}
 
 
