@@ -389,11 +389,14 @@ def evaluate_stream(params_path=DEFAULT_PARAMS, points=(9.0, 9.2, 9.4), frames=3
 
 
 def train_set(codes="s2x", steps=200, batch=512, pool_frames=1024, out_dir=LEARNED_DIR, margin=0.3,
-              eval_frames=2048, force=False, only=None, log=print):
+              eval_frames=2048, force=False, only=None, log=print, code=None, lr_alpha=0.005, lr_beta=0.02,
+              alpha0=1.0, beta0=1.0):
     """Train one parameter file per (frame size, MODCOD) of a mixed stream
     (codes: "mixed12" or "s2x"), each over Es/N0 in [threshold - 0.2,
     threshold + margin + 0.2], then evaluate it at threshold + margin
-    against the plain decoder on the same LLRs.  The file records the
+    against the plain decoder on the same LLRs.  only: modulations to
+    train; code: one (frame size, rate, modulation) to train; lr_alpha,
+    lr_beta, alpha0, beta0 as in train().  The file records the
     evaluation and "use": the learned decoder with plain fallback decoded
     no more frames wrongly than the plain one, in fewer mean iterations."""
     mods = MODCODS12 + (S2X if codes == "s2x" else [])
@@ -403,6 +406,8 @@ def train_set(codes="s2x", steps=200, batch=512, pool_frames=1024, out_dir=LEARN
     done = []
     for rate, mod, th in mods:
         for fs in ((0, 1) if has_short(rate, mod) else (0,)):
+            if code is not None and (fs, rate, mod) != tuple(code):
+                continue
             path = os.path.join(out_dir, params_name(fs, rate, mod))
             have = [f for f in glob.glob(os.path.join(out_dir, "*.json"))
                     if _code_of(f) == (fs, rate, mod)]
@@ -413,7 +418,8 @@ def train_set(codes="s2x", steps=200, batch=512, pool_frames=1024, out_dir=LEARN
             log("train %s %s %s: Es/N0 %.2f..%.2f dB" % ("normal" if fs == 0 else "short", mod, rate,
                                                          th - 0.2, th + margin + 0.2))
             d = train(fs, rate, mod, batch=batch, steps=steps, es_lo=th - 0.2, es_hi=th + margin + 0.2,
-                      pool_frames=pool_frames, log=lambda s: None)
+                      pool_frames=pool_frames, log=lambda s: None, lr_alpha=lr_alpha, lr_beta=lr_beta,
+                      alpha0=alpha0, beta0=beta0)
             rows = evaluate(points=(th + margin,), frames=eval_frames, pool_frames=pool_frames,
                             log=lambda s: None, params=d)
             r = {x["decoder"]: x for x in rows}
@@ -469,6 +475,11 @@ def main(argv=None):
     ts.add_argument("--out-dir", default=LEARNED_DIR)
     ts.add_argument("--only", default=None, help="comma-separated modulations to train")
     ts.add_argument("--force", action="store_true", help="retrain codes that have a file")
+    ts.add_argument("--code", default=None, help="one code to train: normal|short,RATE,MODULATION")
+    ts.add_argument("--lr-alpha", type=float, default=0.005)
+    ts.add_argument("--lr-beta", type=float, default=0.02)
+    ts.add_argument("--alpha0", type=float, default=1.0, help="starting scale of every (iteration, layer)")
+    ts.add_argument("--beta0", type=float, default=1.0, help="starting offset of every (iteration, layer)")
     e = sub.add_parser("eval")
     e.add_argument("--params", default=DEFAULT_PARAMS)
     e.add_argument("--frames", type=int, default=16384)
@@ -490,8 +501,13 @@ def main(argv=None):
             json.dump(d, f, indent=1)
         log("wrote " + a.out)
     elif a.cmd == "train-set":
+        code = None
+        if a.code:
+            fs, rate, mod = a.code.split(",")
+            code = ({"normal": 0, "short": 1}[fs], rate, mod)
         train_set(a.codes, a.steps, a.batch, a.pool, a.out_dir, eval_frames=a.eval_frames, force=a.force,
-                  only=a.only.split(",") if a.only else None, log=log)
+                  only=a.only.split(",") if a.only else None, log=log, code=code, lr_alpha=a.lr_alpha,
+                  lr_beta=a.lr_beta, alpha0=a.alpha0, beta0=a.beta0)
     else:
         pts = [float(x) for x in a.points.split(",")]
         if a.stream:
