@@ -245,6 +245,7 @@ class FecContext:
 
 LEARN_QUANTIZED = 0x1
 STREAM_LEARNED = 0x1
+STREAM_NO_PAIRS = 0x2
 STREAM_ORDER_GIVEN, STREAM_ORDER_COST = 0, 1
 STREAM_FRAME = np.dtype([("code", np.int32), ("reserved", np.int32), ("llr_offset", np.int64),
                          ("hard_offset", np.int64)])

@@ -55,6 +55,9 @@ int ldpc_scratch_setup();
 bool ldpc_split_available(int cmax, bool exact, int lanes);
 int ldpc_lds_bytes_split(int K, int R, int n_slots, int n_steps, int n_groups, int lanes);
 size_t ldpc_msg_bytes_split(int q, int cmax);
+// split-edge kernel decoding two frames of a one-lane code at once (one
+// per lane): two frame regions, then the shared tables and wg_any2 flags
+int ldpc_lds_bytes_pair(int K, int R, int n_slots, int n_steps, int n_groups);
 template <int CMAX, bool EXACT, int LANES>
 hipError_t launch_ldpc_split_variant(const ldpc_args& a, int batch, hipStream_t stream);
 template <int CMAX, bool EXACT, int LANES>

@@ -77,6 +77,15 @@ size_t ldpc_msg_bytes_split(int q, int cmax)
     return (size_t)q * 2 * 180 * words * 4;
 }
 
+int ldpc_lds_bytes_pair(int K, int R, int n_slots, int n_steps, int n_groups)
+{
+    // two frame regions (APP, parity, delta slots), the exchange area of
+    // both lanes, the per-(step, lane) entries and learned parameters, the
+    // group records and the [3][2] flags of wg_any2
+    const int frame = (K + R) * 2 + n_slots * CIRC * 2;
+    return (2 * frame + 3 * 2 * 2 * 180 * 4 + 48 * n_steps + 4 * (8 * n_groups + 6) + 15) & ~15;
+}
+
 hipError_t launch_ldpc_decode(const ldpc_args& a, int batch, hipStream_t stream)
 {
     // the split kernel only where its LDS fits a workgroup (the 6-wave

@@ -168,6 +168,24 @@ __device__ __forceinline__ int wg_any(int v, int* flags, int& k)
     return r;
 }
 
+// wg_any over two groups of threads (g = 0, 1; flags [3][2]): bit g of the
+// result is set if any thread of group g passed v != 0.  The same value in
+// every thread of the workgroup.
+__device__ __forceinline__ int wg_any2(int v, int g, int* flags, int& k)
+{
+    const int slot = k % 3;
+    if (v) flags[2 * slot + g] = 1;
+    if (threadIdx.x == 0) {
+        const int nx = slot == 2 ? 0 : slot + 1;
+        flags[2 * nx] = 0;
+        flags[2 * nx + 1] = 0;
+    }
+    __syncthreads();
+    const int r = (flags[2 * slot] ? 1 : 0) | (flags[2 * slot + 1] ? 2 : 0);
+    k++;
+    return r;
+}
+
 // Collision group record (int32 x 8, host-packed): LDS dword index of the
 // VN group's first pair dword, number of slots, the slots' dword indices.
 constexpr int GROUP_WORDS = 8;

@@ -65,7 +65,14 @@ __device__ __forceinline__ void group_apply(uint32_t* app32, const int* rec, int
     app32[base + tt] = bits(val);
 }
 
-template <int CMAX, bool EXACT, int LANES>
+// One frame per workgroup (PAIR = false), or, for a code with a one-lane
+// schedule, two frames of that code at once (PAIR = true, LANES = 2): lane
+// L decodes frame L (llr / rec_base / h, then llr1 / rec1 / h1) in its own
+// LDS region, both lanes step through the shared schedule together, and a
+// lane whose frame has met its syndrome idles until the other's is done.
+// Returns the frame's trial count (PAIR: frame 0's in the low 16 bits,
+// frame 1's in the high 16 bits, each an int16).
+template <int CMAX, bool EXACT, int LANES, bool PAIR = false>
 __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const int16_t* __restrict__ llr,
                                                   uint4* __restrict__ rec_base, uint8_t* __restrict__ h,
                                                   const circ_rec* __restrict__ circ,
@@ -73,14 +80,30 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                                                   const int* __restrict__ steps, const int* __restrict__ grp_off,
                                                   const int* __restrict__ step_defer,
                                                   const int* __restrict__ groups, const int* __restrict__ beta,
-                                                  const int* __restrict__ alpha, const kparams& p)
+                                                  const int* __restrict__ alpha, const kparams& p,
+                                                  const int16_t* __restrict__ llr1 = nullptr,
+                                                  uint4* __restrict__ rec1 = nullptr, uint8_t* __restrict__ h1 = nullptr)
 {
+    static_assert(!PAIR || LANES == 2, "a pair takes both lanes");
     using R = rec_half_t<CMAX>;
     constexpr int NE0 = R::NE0, NE1 = R::NE1, NEH = R::NEH;
-    uint32_t* app32 = reinterpret_cast<uint32_t*>(lds_raw);  // K/2 dwords
-    uint32_t* par32 = app32 + p.K / 2;                        // R/2 dwords
-    uint32_t* dsl32 = par32 + p.R / 2;                        // n_slots * 180 dwords
-    uint32_t* xch = dsl32 + p.n_slots * HALF;                 // [3][LANES][2][180] partials
+    constexpr int NTHR = LANES * SPLIT_LANE_THREADS;
+    const int t = threadIdx.x;
+    const int lane = LANES == 2 ? __builtin_amdgcn_readfirstlane(t >= SPLIT_LANE_THREADS ? 1 : 0) : 0;
+    // PAIR: frame region of lane L at L * frw dwords (its APP, parity and
+    // slot areas, laid out as for one frame); the shared tables after both
+    const int frw = PAIR ? p.K / 2 + p.R / 2 + p.n_slots * HALF : 0;
+    unsigned char* lds_f = PAIR ? lds_raw + lane * frw * 4 : lds_raw;  // this lane's frame region
+    if constexpr (PAIR) {
+        llr = lane ? llr1 : llr;
+        rec_base = lane ? rec1 : rec_base;
+        h = lane ? h1 : h;
+    }
+    uint32_t* app32 = reinterpret_cast<uint32_t*>(lds_f);  // K/2 dwords
+    uint32_t* par32 = app32 + p.K / 2;                      // R/2 dwords
+    uint32_t* dsl32 = par32 + p.R / 2;                      // n_slots * 180 dwords
+    uint32_t* xch = PAIR ? reinterpret_cast<uint32_t*>(lds_raw) + 2 * frw
+                         : dsl32 + p.n_slots * HALF;        // [3][LANES][2][180] partials
     // per (step, lane): layer (-1 idle), its lay_off and data-circulant
     // count, and the step's first collision group, so that a step's first
     // global access is its circulant records
@@ -89,12 +112,9 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
     // written during the iteration's syndrome pass
     int2* parl = reinterpret_cast<int2*>(sched_l + 2 * p.n_steps);
     int* grp_l = reinterpret_cast<int*>(parl + 2 * p.n_steps);  // [n_groups][8], 16-byte aligned
-    int* flg = grp_l + GROUP_WORDS * p.n_groups;  // [3] wg_any flags
+    int* flg = grp_l + GROUP_WORDS * p.n_groups;  // [3] wg_any flags (PAIR: [3][2])
     int16_t* par16 = reinterpret_cast<int16_t*>(par32);
 
-    constexpr int NTHR = LANES * SPLIT_LANE_THREADS;
-    const int t = threadIdx.x;
-    const int lane = LANES == 2 ? __builtin_amdgcn_readfirstlane(t >= SPLIT_LANE_THREADS ? 1 : 0) : 0;
     const int u = t - lane * SPLIT_LANE_THREADS;
     const int half = __builtin_amdgcn_readfirstlane(u >= LANE_THREADS ? 1 : 0);
     const int tt = u - half * LANE_THREADS;
@@ -103,6 +123,16 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
     const bool act = tt < HALF;
     const int q = p.q;
     const int quarter = lane * 2 + half;  // which (lane, half) this wave group is
+    // the schedule entry of this lane (PAIR: both lanes run lane 0's layers)
+    const int sln = PAIR ? 0 : lane;
+    // collision groups: wave group gq of gs applies groups gq, gq + gs, ...
+    // of the frame (PAIR: the two halves of each lane, on the lane's frame)
+    const int gq = PAIR ? half : quarter;
+    constexpr int gs = PAIR ? 2 : 2 * LANES;
+    // PAIR: bit L set = frame L has met its syndrome (the same in every
+    // thread); dn = this lane's frame is done and it only keeps step
+    int dmask = 0;
+    bool dn = false;
     uint32_t* xm1 = xch;
     uint32_t* xm2 = xch + LANES * 2 * HALF;
     uint32_t* xsg = xch + 2 * LANES * 2 * HALF;
@@ -115,10 +145,13 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                                grp_off[k >> 1] | (step_defer[k >> 1] ? 1 << 30 : 0));
     }
     for (int k = t; k < GROUP_WORDS * p.n_groups; k += NTHR) grp_l[k] = groups[k];
-    if (t < 3) flg[t] = 0;
+    if (t < (PAIR ? 6 : 3)) flg[t] = 0;
     int any_k = 0;
     const int n_groups_all = p.n_groups;  // first group past the last step
-    load_frame_llr<NTHR>(app32, par16, llr, p.K, p.R, q, t);
+    if constexpr (PAIR)
+        load_frame_llr<SPLIT_LANE_THREADS>(app32, par16, llr, p.K, p.R, q, u);
+    else
+        load_frame_llr<NTHR>(app32, par16, llr, p.K, p.R, q, t);
     __syncthreads();
 
     // Message records are not cleared: iteration 0 reads zero_records
@@ -137,7 +170,7 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
     uint4 ra[R::V4], rb[R::V4];
     int si[2], so[2], sc[2], sg0[2];  // layer, lay_off, circulant count, first group of the step in ra / rb
     {
-        const int4 e0 = sched_l[lane];
+        const int4 e0 = sched_l[sln];
         si[0] = __builtin_amdgcn_readfirstlane(e0.x);
         so[0] = __builtin_amdgcn_readfirstlane(e0.y);
         sc[0] = __builtin_amdgcn_readfirstlane(e0.z);
@@ -153,7 +186,7 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
         }
     }
 
-    int res = -1;
+    int res = -1, res1 = -1;  // res1: frame 1 of a pair
     for (int it = 0;; it++) {
         const bool lrn = it < p.n_param_iters;  // wave-uniform
         int pbv = 1, pav = 0;
@@ -167,14 +200,17 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
         // ---- syndrome of the APP signs, early exit (ldpc_decode_kernel.cu:689-749) ----
         // Per barrier, two schedule steps: wave group (lane, half) evaluates
         // the whole check pairs of its lane's layer in step s + half.  The
-        // result is the same for any order of the checks.
+        // result is the same for any order of the checks.  PAIR: wave group
+        // (lane, half) checks step s + half of its lane's frame; the scan
+        // ends once each frame is done or has a failed check.
         int ok = 1;
+        int failm = 0;  // PAIR: bit L = frame L has a failed check
         for (int s = 0; s < p.n_steps; s += 2) {
             const int sx = min(s + half, p.n_steps - 1);
-            const int4 se = sched_l[2 * sx + lane];
+            const int4 se = sched_l[2 * sx + sln];
             const int i = __builtin_amdgcn_readfirstlane(se.x);
             int bad = 0;
-            if (act && i >= 0 && s + half < p.n_steps) {
+            if (act && !dn && i >= 0 && s + half < p.n_steps) {
                 uint32_t sg = par32[i * HALF + tt];
                 if (i > 0) {
                     sg ^= par32[(i - 1) * HALF + tt];
@@ -191,12 +227,17 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                         int off;
                         uint32_t sel;
                         pair_addr(c, tt4, tt, off, sel);
-                        sg ^= perm(lds_ld(lds_raw, c.app_off + off), sel);
+                        sg ^= perm(lds_ld(lds_f, c.app_off + off), sel);
                     }
                 }
                 bad = (sg & 0x80008000u) != 0;
             }
-            if (wg_any(bad, flg, any_k)) { ok = 0; break; }
+            if constexpr (PAIR) {
+                failm |= wg_any2(bad, lane, flg, any_k);
+                if ((failm | dmask) == 3) break;  // workgroup-uniform
+            } else {
+                if (wg_any(bad, flg, any_k)) { ok = 0; break; }
+            }
         }
         // the learned parameters, loaded before the scan: read at the
         // exchanges, after the first step's barrier
@@ -205,7 +246,16 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
         // parameter loads to the store's branch and would otherwise count
         // them as outstanding in the step loop, waiting at every step
         __builtin_amdgcn_s_waitcnt(0x0F70);
-        if (ok) { res = p.max_trials - it; break; }
+        if constexpr (PAIR) {
+            const int okm = ~(failm | dmask) & 3;  // frames that meet their syndrome now
+            if (okm & 1) res = p.max_trials - it;
+            if (okm & 2) res1 = p.max_trials - it;
+            dmask |= okm;
+            dn = (dmask >> lane) & 1;
+            if (dmask == 3) break;
+        } else {
+            if (ok) { res = p.max_trials - it; break; }
+        }
         if (it == p.max_trials) break;
 
         // ---- one layered iteration ----
@@ -233,7 +283,7 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
             }
             {
                 const int sn = s + 1 < p.n_steps ? s + 1 : 0;
-                const int4 en = sched_l[2 * sn + lane];
+                const int4 en = sched_l[2 * sn + sln];
                 si[hs ^ 1] = __builtin_amdgcn_readfirstlane(en.x);
                 so[hs ^ 1] = __builtin_amdgcn_readfirstlane(en.y);
                 sc[hs ^ 1] = __builtin_amdgcn_readfirstlane(en.z);
@@ -245,7 +295,7 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
 #pragma unroll
                 for (int k = 0; k < R::V4; k++) nxt[k] = src[k];
             }
-            const bool work = act && i >= 0;
+            const bool work = act && !dn && i >= 0;
             const s2 zero = splat(0);
             // data edges of the layer and of this half (wave-uniform)
             const int cnt = EXACT ? CMAX : lcnt;
@@ -264,15 +314,15 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
             const int pidx = half == 0 ? i * HALF + tt
                                        : (i > 0 ? (i - 1) * HALF + tt : (q - 1) * HALF + (tt > 0 ? tt - 1 : HALF - 1));
             // Deferred groups of the previous step: their rows are untouched by
-            // this step, so wave group `quarter` applies group prev_g0 + quarter
-            // alongside the step -- reads issued here, the result written
-            // before the exchange barrier (more groups than wave groups: the
-            // rest after it, synchronously).
-            const bool mat = prev_def && prev_g0 + quarter < prev_g1;  // wave-uniform
+            // this step, so wave group gq applies group prev_g0 + gq alongside
+            // the step -- reads issued here, the result written before the
+            // exchange barrier (more groups than wave groups: the rest after
+            // it, synchronously).
+            const bool mat = prev_def && !dn && prev_g0 + gq < prev_g1;  // wave-uniform
             int m_base = 0, m_n = 0;
             uint32_t m_a0 = 0, m_d[6] = {0, 0, 0, 0, 0, 0};
             if (mat) {
-                const int g = prev_g0 + quarter;
+                const int g = prev_g0 + gq;
                 const int4 ga = *reinterpret_cast<const int4*>(grp_l + GROUP_WORDS * g);
                 const int4 gb = *reinterpret_cast<const int4*>(grp_l + GROUP_WORDS * g + 4);
                 m_base = __builtin_amdgcn_readfirstlane(ga.x);
@@ -304,7 +354,7 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                         const int off = wrap ? tt4w : tt4;
                         sels[e] = (wrap != (c.sel != SEL_ID)) ? SEL_ROT : SEL_ID;
                         ra_[e] = c.app_off + off;
-                        apw[e] = lds_ld(lds_raw, ra_[e]);
+                        apw[e] = lds_ld(lds_f, ra_[e]);
                         slotm |= (c.slot_off < 0 ? 0u : 1u) << e;
                     }
                 }
@@ -319,7 +369,7 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                             const int off = wrap ? tt4w : tt4;
                             sels[e] = (wrap != (c.sel != SEL_ID)) ? SEL_ROT : SEL_ID;
                             ra_[e] = c.app_off + off;
-                            apw[e] = lds_ld(lds_raw, ra_[e]);
+                            apw[e] = lds_ld(lds_f, ra_[e]);
                             slotm |= (c.slot_off < 0 ? 0u : 1u) << e;
                         }
                         const s2 ap = as_s2(perm(apw[e], sels[e]));
@@ -358,8 +408,8 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                 app32[m_base + tt] = bits(val);
             }
             __syncthreads();  // every read of the step is done; partials are in LDS
-            if (prev_def && act)
-                for (int g = prev_g0 + quarter + 2 * LANES; g < prev_g1; g += 2 * LANES)
+            if (prev_def && act && !dn)
+                for (int g = prev_g0 + gq + gs; g < prev_g1; g += gs)
                     group_apply(app32, grp_l + GROUP_WORDS * g, tt);
             if (work) {
                 {
@@ -372,7 +422,7 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                 }
                 int bt = 1, al = 0;
                 if (lrn) {
-                    const int2 pr = parl[2 * s + lane];
+                    const int2 pr = parl[2 * s + sln];
                     bt = __builtin_amdgcn_readfirstlane(pr.x);
                     al = __builtin_amdgcn_readfirstlane(pr.y);
                 }
@@ -413,7 +463,7 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                         if (e < ne) {
                             const s2 y = out(mv[e], xv[e]);
                             ys[e] = y;
-                            lds_st(lds_raw, ra_[e],
+                            lds_st(lds_f, ra_[e],
                                    perm(bits(__builtin_elementwise_add_sat(av[e], y - ov[e])), sels[e]));
                         }
                     }
@@ -427,9 +477,9 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                             const s2 d = y - ov[e];
                             if ((slotu >> e) & 1u) {
                                 const circ_rec c = cc[e];
-                                lds_st(lds_raw, ra_[e] + (c.slot_off - c.app_off), perm(bits(d), sels[e]));
+                                lds_st(lds_f, ra_[e] + (c.slot_off - c.app_off), perm(bits(d), sels[e]));
                             } else {
-                                lds_st(lds_raw, ra_[e],
+                                lds_st(lds_f, ra_[e],
                                        perm(bits(__builtin_elementwise_add_sat(av[e], d)), sels[e]));
                             }
                         }
@@ -471,9 +521,9 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                 // of pair dword tt in slot order.  The group record comes in
                 // two 16-byte reads and every slot read is issued before the
                 // first add, so a group costs two LDS round trips.
-                for (int gb = g0; gb < g1; gb += 2 * LANES) {
-                    const int g = gb + quarter;
-                    if (g < g1 && act) group_apply(app32, grp_l + GROUP_WORDS * g, tt);
+                for (int gb = g0; gb < g1; gb += gs) {
+                    const int g = gb + gq;
+                    if (g < g1 && act && !dn) group_apply(app32, grp_l + GROUP_WORDS * g, tt);
                 }
                 __syncthreads();
             }
@@ -481,8 +531,14 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
         }
     }
 
-    store_hard<NTHR>(app32, h, p.K, t);
-    return res;
+    if constexpr (PAIR) {
+        // a done lane's APP has not changed since its frame met its syndrome
+        store_hard<SPLIT_LANE_THREADS>(app32, h, p.K, u);
+        return (int)(((uint32_t)res & 0xffffu) | ((uint32_t)res1 << 16));
+    } else {
+        store_hard<NTHR>(app32, h, p.K, t);
+        return res;
+    }
 }
 
 template <int CMAX, bool EXACT, int LANES>

@@ -68,6 +68,7 @@ struct code_desc {
     int variant;        // decode_frame variant (CUSAT_STREAM_VARIANTS id)
     int variant_split;  // decode_frame_split variant (CUSAT_STREAM_SPLIT_VARIANTS id), or -1
     int bch_bytes, bch_degree, bch_chunk, bch_words;
+    int pair;  // 1: the 12-wave kernel decodes two frames of this code at once (one-lane schedule)
 };
 
 // two-lane decoder variants in the stream kernel: (id, CMAX, exact)
@@ -80,6 +81,22 @@ struct code_desc {
 // two-lane split-edge variants of the 12-wave kernel: (id, CMAX, exact), the
 // same set (every code the stream takes has one)
 #define CUSAT_STREAM_SPLIT_VARIANTS(X) CUSAT_STREAM_VARIANTS(X)
+
+// split-edge variants with a two-frame form: those of the codes with a
+// one-lane schedule (the short FECFRAMEs: at most 25 info circulants per
+// layer, the guarded ones at most 17)
+#define CUSAT_STREAM_PAIR_VARIANTS(X)                                                                \
+    X(0, 2, true) X(1, 3, true) X(2, 4, true) X(3, 5, true) X(4, 7, true) X(5, 8, true)             \
+    X(6, 9, true) X(7, 11, true) X(11, 25, true) X(13, 6, false) X(14, 12, false) X(15, 18, false)
+
+bool pair_variant(int id)
+{
+#define X(ID, C, E) \
+    if (id == ID) return true;
+    CUSAT_STREAM_PAIR_VARIANTS(X)
+#undef X
+    return false;
+}
 
 int variant_id(int cmax, bool exact)
 {
@@ -143,6 +160,7 @@ struct code_desc_dev {
     int variant;
     int variant_split;
     int bch_bytes, bch_degree, bch_chunk, bch_words;
+    int pair;
 };
 static_assert(sizeof(code_desc_dev) == sizeof(code_desc), "code_desc_dev mirrors code_desc");
 
@@ -218,6 +236,49 @@ __device__ __forceinline__ int decode_variant(const code_desc* codes_g, int ci, 
 #undef X
     }
     return -1;
+}
+
+// Two frames of a code with a one-lane schedule at once, one per lane
+// (decode_frame_split's PAIR form), as a function of its own as decode_one.
+// Returns frame 0's trial count in the low 16 bits, frame 1's in the high
+// 16 bits.
+template <int C, bool E, bool FIXED>
+__device__ __attribute__((noinline)) int decode_two(CUSAT_AS_CONST const code_desc_dev* codes_v, int ci_v,
+                                                    CUSAT_AS_CONST const int16_t* llr0_v, CUSAT_AS_GLOBAL uint4* rec0_v,
+                                                    CUSAT_AS_GLOBAL uint8_t* h0_v,
+                                                    CUSAT_AS_CONST const int16_t* llr1_v, CUSAT_AS_GLOBAL uint4* rec1_v,
+                                                    CUSAT_AS_GLOBAL uint8_t* h1_v, int lp_v)
+{
+    const auto& c = uniform(codes_v)[__builtin_amdgcn_readfirstlane(ci_v)];
+    const bool lp = __builtin_amdgcn_readfirstlane(lp_v) != 0;
+    const ldpc_dev::kparams p{c.N,      c.K, c.R, c.q, c.n_steps, lp ? c.learned_trials : c.max_trials,
+                              lp ? c.n_param_iters : 0, c.n_slots, c.n_groups};
+    const auto* circ = (const ldpc_dev::circ_rec*)c.circ;
+    const int* beta = lp ? (const int*)c.beta : nullptr;
+    const int* alpha = lp ? (const int*)c.alpha : nullptr;
+    return ldpc_dev::decode_frame_split<C, E, 2, true>(
+        stream_lds<FIXED>(), (const int16_t*)uniform(llr0_v), (uint4*)uniform(rec0_v), (uint8_t*)uniform(h0_v), circ,
+        (const int*)c.lay_cnt, (const int*)c.lay_off, (const int*)c.steps, (const int*)c.grp_off,
+        (const int*)c.step_defer, (const int*)c.groups, beta, alpha, p, (const int16_t*)uniform(llr1_v),
+        (uint4*)uniform(rec1_v), (uint8_t*)uniform(h1_v));
+}
+
+template <bool FIXED>
+__device__ __forceinline__ int decode_variant_two(const code_desc* codes_g, int ci, const code_desc& c,
+                                                  const int16_t* llr0, uint4* rec0, uint8_t* h0,
+                                                  const int16_t* llr1, uint4* rec1, uint8_t* h1, int lp)
+{
+    const auto* codes = (CUSAT_AS_CONST const code_desc_dev*)codes_g;
+    switch (c.variant_split) {
+#define X(ID, C, E)                                                                                                \
+    case ID:                                                                                                       \
+        return decode_two<C, E, FIXED>(codes, ci, (CUSAT_AS_CONST const int16_t*)llr0, (CUSAT_AS_GLOBAL uint4*)rec0, \
+                                       (CUSAT_AS_GLOBAL uint8_t*)h0, (CUSAT_AS_CONST const int16_t*)llr1,           \
+                                       (CUSAT_AS_GLOBAL uint4*)rec1, (CUSAT_AS_GLOBAL uint8_t*)h1, lp);
+        CUSAT_STREAM_PAIR_VARIANTS(X)
+#undef X
+    }
+    return -1;  // both halves -1 (the host never marks another variant as a pair)
 }
 
 // c(x) mod g(x) == 0 over the frame's n_bytes hard bytes, by wave 0 (the
@@ -420,11 +481,42 @@ __global__ void __launch_bounds__(stream_threads<WIDE>())
             const code_desc& c = codes[ci];
             const int16_t* fl = llr + frames[f].llr_offset;
             uint8_t* fh = hard + frames[f].hard_offset;
+            const bool use_learned = learned && c.learned_trials > 0;
+            if constexpr (WIDE && FIXED) {
+                if (c.pair && k + 1 < J.y) {
+                    // two frames of the job (one code) at once, one per lane;
+                    // frame 1's message records in the workspace's second half
+                    const int f1 = __builtin_amdgcn_readfirstlane(order[J.x + k + 1]);
+                    const int two = decode_variant_two<FIXED>(codes, ci, c, fl, rec, fh, llr + frames[f1].llr_offset,
+                                                              rec + ws_v4 / 2, hard + frames[f1].hard_offset,
+                                                              use_learned ? 1 : 0);
+                    __syncthreads();
+                    for (int m = 0; m < 2; m++) {
+                        const int fm = m ? f1 : f;
+                        int res = m ? two >> 16 : (int)(int16_t)(two & 0xffff);
+                        const int fb = use_learned && res < 0 ? 1 : 0;
+                        if (fb) {
+                            res = decode_variant<WIDE, FIXED>(codes, ci, c, llr + frames[fm].llr_offset, rec,
+                                                              hard + frames[fm].hard_offset, 0);
+                            __syncthreads();
+                        }
+                        if (threadIdx.x == 0) {
+                            trials[fm] = res;
+                            if (fallback) fallback[fm] = fb;
+                            ctl.frame[npend] = fm;
+                            ctl.res[npend] = res;
+                            ctl.fb[npend] = fb;
+                        }
+                        if (++npend == PEND) bch_round();
+                    }
+                    k++;
+                    continue;
+                }
+            }
             // learned pass, then the plain decoder at once when the
             // syndrome is not met (a barrier after each decode: its last
             // step packs the hard decisions from the LDS that the next
             // decode starts writing)
-            const bool use_learned = learned && c.learned_trials > 0;
             int res = decode_variant<WIDE, FIXED>(codes, ci, c, fl, rec, fh, use_learned ? 1 : 0);
             __syncthreads();
             const int fb = use_learned && res < 0 ? 1 : 0;
@@ -468,6 +560,7 @@ struct cusat_stream_ctx {
     std::vector<cusat_fec_ctx*> codes;
     std::vector<int> variant;        // stream decoder variant of each code
     std::vector<int> variant_split;  // its split-edge variant, or -1
+    std::vector<int> pair;           // 1: two frames per workgroup (code_desc::pair)
     int max_frames = 0;
     unsigned flags = 0;
     int n_cu = 0;
@@ -534,16 +627,20 @@ int stage_plan(const cusat_stream_ctx* sc, const cusat_stream_tuning& tu, unsign
     int* order = reinterpret_cast<int*>(img + sc->off_order);
     int2* jobs = reinterpret_cast<int2*>(img + sc->off_jobs);
     std::iota(order, order + n, 0);
+    // codes of equal cost kept apart, so that a code's frames are neighbours
     if (tu.order == CUSAT_STREAM_ORDER_COST)
         std::stable_sort(order, order + n, [&](int a, int b) {
-            return frame_cost(sc->codes[sc->frames[a].code]) > frame_cost(sc->codes[sc->frames[b].code]);
+            const int ca = sc->frames[a].code, cb = sc->frames[b].code;
+            const int xa = frame_cost(sc->codes[ca]), xb = frame_cost(sc->codes[cb]);
+            return xa != xb ? xa > xb : ca < cb;
         });
     int nj = 0;
     for (int k = 0; k < n;) {
+        // a code decoded two frames at a time takes twice the frames per job
+        const int code = sc->frames[order[k]].code;
+        const int per = tu.split_edges && sc->pair[code] ? 2 * tu.frames_per_job : tu.frames_per_job;
         int m = 1;
-        while (m < tu.frames_per_job && k + m < n &&
-               sc->frames[order[k + m]].code == sc->frames[order[k]].code)
-            m++;
+        while (m < per && k + m < n && sc->frames[order[k + m]].code == code) m++;
         jobs[nj++] = make_int2(k, m);
         k += m;
     }
@@ -615,6 +712,7 @@ void fill_code_descs(const cusat_stream_ctx* sc, code_desc* d)
         o.bch_degree = c->bch.degree;
         o.bch_chunk = c->bch_chunk;
         o.bch_words = (c->bch.degree + 63) / 64;
+        o.pair = sc->pair[k];
     }
 }
 
@@ -761,6 +859,7 @@ extern "C" int cusat_stream_create(cusat_stream_ctx** out, cusat_fec_ctx* const*
         sc->variant.push_back(variant_id(cmax, exact));
         const int vs = split_variant_id(cmax, exact);
         sc->variant_split.push_back(vs);
+        sc->pair.push_back(0);
         rec_max = std::max(rec_max, ldpc_msg_bytes(c->geo.q, cmax));
         const int ng = (int)c->geo.groups.size();
         const int lds = ldpc_lds_bytes(c->geo.K, c->geo.R, c->geo.n_slots, c->kern_steps, ng);
@@ -773,6 +872,29 @@ extern "C" int cusat_stream_create(cusat_stream_ctx** out, cusat_fec_ctx* const*
         sc->lds[0] = std::max(sc->lds[0], std::max(lds, 256 * 4 * 8));
         sc->lds[1] = std::max(sc->lds[1], std::max(lds_w, 256 * 4 * 8));
     }
+    // Two frames per workgroup for the codes with a one-lane schedule, when
+    // the 12-wave kernel holds one workgroup per CU anyway (its LDS is more
+    // than half a CU's) and two such frames fit in that LDS: lane 1's six
+    // waves would otherwise idle through those frames.  The message
+    // workspace then holds two frames' records.
+    if (!(flags & CUSAT_STREAM_NO_PAIRS) && sc->lds[1] > ldpc_dev::SPLIT_LDS_STATIC / 2)
+        for (int k = 0; k < n_codes; k++) {
+            const cusat_fec_ctx* c = sc->codes[k];
+            if (c->geo.lanes != 1 || !pair_variant(sc->variant_split[k])) continue;
+            const int lp = ldpc_lds_bytes_pair(c->geo.K, c->geo.R, c->geo.n_slots, c->kern_steps,
+                                               (int)c->geo.groups.size());
+            if (lp > sc->lds[1]) continue;
+            int cmax = 0;
+            bool exact = false;
+            int cmin = 1 << 30, cmx = 0;
+            for (int v : c->geo.lay_cnt) {
+                cmin = std::min(cmin, v);
+                cmx = std::max(cmx, v);
+            }
+            ldpc_select_variant(cmin, cmx, 2, &cmax, &exact);  // as above: it succeeded there
+            sc->pair[k] = 1;
+            rec_max = std::max(rec_max, 2 * ldpc_msg_bytes_split(c->geo.q, cmax));
+        }
     for (int w = 0; w < 2; w++) sc->lds[w] += CTL_BYTES;  // control area
     int dev = 0;
     hipGetDevice(&dev);

@@ -256,6 +256,12 @@ typedef struct cusat_stream_tuning {
  * then every frame failing frame_passes_fec_checks is decoded again by the
  * plain decoder (the context's max_trials) inside the same launch */
 #define CUSAT_STREAM_LEARNED 0x1u
+/* stream flag: never decode two short frames in one workgroup.  By default,
+ * when the 12-wave kernel runs one workgroup per CU (a stream with normal
+ * frames), a code with a one-lane schedule (short frames) decodes two of its
+ * frames at once, one per lane, whenever a job holds two: the results are
+ * the same either way */
+#define CUSAT_STREAM_NO_PAIRS 0x2u
 /* contexts must outlive the stream context; max_frames bounds a plan */
 int cusat_stream_create(cusat_stream_ctx** out, cusat_fec_ctx* const* codes, int n_codes, int max_frames,
                         unsigned flags);

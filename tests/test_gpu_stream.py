@@ -147,6 +147,35 @@ def test_stream_learned_matches_rx_fec_learned():
         check(ctxs, frames, who, ref, run_stream(sc, frames, llr, hb))
 
 
+def test_stream_pairs_of_short_frames():
+    """Next to normal frames the 12-wave kernel decodes two short frames of a
+    code per workgroup, one per lane: every frame must still equal its own
+    context's decode (learned with and without fallback, frames that fail,
+    an odd frame count per code) and the unpaired stream's."""
+    from cusat_amd import learn
+    mix = [(0, "3/4", "8PSK", 7.9), (1, "1/2", "QPSK", 1.9), (1, "3/5", "8PSK", 6.2),
+           (1, "5/6", "16APSK", 11.4), (1, "1/4", "QPSK", -0.6), (1, "2/3", "QPSK", 3.3)]
+    per_code = 9
+    ctxs = [ca.FecContext(fs, r, m, max_trials=50, max_batch=per_code) for fs, r, m, _ in mix]
+    path = learn.params_for(1, "1/2", "QPSK")
+    _, bq, aq = learn.load_params(path)
+    ctxs[1].set_params(bq, aq)
+    ctxs[1]._learned = True
+    ctxs, frames, llr, by_code, who, hb = build_stream(mix, per_code, 11, contexts=ctxs)
+    ref = per_code_reference(ctxs, by_code, learned_trials=bq.shape[0])
+    t = np.concatenate([r[1] for r in ref[1:]])
+    assert (t < 0).any() and (t >= 0).any(), "the short codes need failing and converging frames"
+    assert len(set(ref[1][1].tolist())) > 1, "the learned code's frames need different trial counts"
+    for flags in (ca.STREAM_LEARNED, ca.STREAM_LEARNED | ca.STREAM_NO_PAIRS):
+        sc = ca.StreamContext(ctxs, len(frames), flags)
+        for tuning in ({"wg_per_cu": 1, "frames_per_job": 1, "order": 1, "split_edges": 1},
+                       {"wg_per_cu": 1, "frames_per_job": 3, "order": 1, "split_edges": 1},
+                       {"wg_per_cu": 1, "frames_per_job": 2, "order": 0, "split_edges": 1}):
+            sc.plan(frames, tuning)
+            check(ctxs, frames, who, ref, run_stream(sc, frames, llr, hb))
+        sc.close()
+
+
 def test_stream_graph_replays():
     torch = _torch()
     mix = MIX[:3] + MIX[5:6]
