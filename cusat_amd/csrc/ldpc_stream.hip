@@ -634,11 +634,14 @@ int stage_plan(const cusat_stream_ctx* sc, const cusat_stream_tuning& tu, unsign
             const int xa = frame_cost(sc->codes[ca]), xb = frame_cost(sc->codes[cb]);
             return xa != xb ? xa > xb : ca < cb;
         });
+    // a code decoded two frames at a time takes twice the frames per job,
+    // when the plan has more frames than the grid has workgroups (else every
+    // frame has a workgroup of its own, and a pair would only be slower)
+    const bool pairs = tu.split_edges && n > sc->n_cu * tu.wg_per_cu;
     int nj = 0;
     for (int k = 0; k < n;) {
-        // a code decoded two frames at a time takes twice the frames per job
         const int code = sc->frames[order[k]].code;
-        const int per = tu.split_edges && sc->pair[code] ? 2 * tu.frames_per_job : tu.frames_per_job;
+        const int per = pairs && sc->pair[code] ? 2 * tu.frames_per_job : tu.frames_per_job;
         int m = 1;
         while (m < per && k + m < n && sc->frames[order[k + m]].code == code) m++;
         jobs[nj++] = make_int2(k, m);

@@ -258,9 +258,10 @@ typedef struct cusat_stream_tuning {
 #define CUSAT_STREAM_LEARNED 0x1u
 /* stream flag: never decode two short frames in one workgroup.  By default,
  * when the 12-wave kernel runs one workgroup per CU (a stream with normal
- * frames), a code with a one-lane schedule (short frames) decodes two of its
- * frames at once, one per lane, whenever a job holds two: the results are
- * the same either way */
+ * frames) and a plan has more frames than the launch has workgroups, a code
+ * with a one-lane schedule (short frames) decodes two of its frames at once,
+ * one per lane (jobs of such a code hold twice frames_per_job frames): the
+ * results are the same either way */
 #define CUSAT_STREAM_NO_PAIRS 0x2u
 /* contexts must outlive the stream context; max_frames bounds a plan */
 int cusat_stream_create(cusat_stream_ctx** out, cusat_fec_ctx* const* codes, int n_codes, int max_frames,

@@ -153,9 +153,11 @@ def test_stream_pairs_of_short_frames():
     context's decode (learned with and without fallback, frames that fail,
     an odd frame count per code) and the unpaired stream's."""
     from cusat_amd import learn
+    torch = _torch()
     mix = [(0, "3/4", "8PSK", 7.9), (1, "1/2", "QPSK", 1.9), (1, "3/5", "8PSK", 6.2),
            (1, "5/6", "16APSK", 11.4), (1, "1/4", "QPSK", -0.6), (1, "2/3", "QPSK", 3.3)]
-    per_code = 9
+    # pairs are planned only when the frames outnumber the workgroups (one per CU)
+    per_code = (torch.cuda.get_device_properties(0).multi_processor_count // len(mix) + 5) | 1
     ctxs = [ca.FecContext(fs, r, m, max_trials=50, max_batch=per_code) for fs, r, m, _ in mix]
     path = learn.params_for(1, "1/2", "QPSK")
     _, bq, aq = learn.load_params(path)
