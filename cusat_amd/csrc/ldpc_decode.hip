@@ -72,8 +72,10 @@ int ldpc_lds_bytes_split(int K, int R, int n_slots, int n_steps, int n_groups, i
 
 size_t ldpc_msg_bytes_split(int q, int cmax)
 {
+    // rec_half_t<cmax>::WORDS per half record
     const int neh = (cmax + 1) / 2 + 1;
-    const int words = (((neh + 1) / 2) + 3) / 4 * 4;
+    const int used = (neh + 1) / 2;
+    const int words = used <= 2 ? used : (used + 3) / 4 * 4;
     return (size_t)q * 2 * 180 * words * 4;
 }
 

@@ -15,8 +15,9 @@
 //    beta is applied once per check, max(min|x| - beta, 0) being the minimum
 //    of max(|x| - beta, 0) -- and the halves swap these partials through LDS
 //    (one barrier); then each half writes its own edges.
-//  * A message record holds one half's edges: [layer][half][pair] x 16 B
-//    multiples, so a wave still moves consecutive 16-byte records.
+//  * A message record holds one half's edges: [layer][half][pair] x 4, 8 or
+//    a multiple of 16 bytes (the bytes its edges take, 2 per edge, rounded
+//    up to a load size), so a wave moves consecutive records.
 //
 // Reads of a step see the step-start APP and every variable node is written
 // once per step outside the collision slots, as in ldpc_kernel.h, so the
@@ -31,15 +32,54 @@ namespace ldpc_dev {
 constexpr int SPLIT_LANE_THREADS = 2 * LANE_THREADS;  // 6 waves per lane
 constexpr int SPLIT_LDS_STATIC = 160 * 1024;           // the whole LDS of a gfx950 workgroup
 
-// message record of one half: (CMAX + 1) / 2 data slots + 1 parity slot
+// message record of one half: (CMAX + 1) / 2 data slots + 1 parity slot,
+// two edges per word; stored as 1 or 2 words when those hold every edge,
+// else as a multiple of 4 words (16-byte loads)
 template <int CMAX>
 struct rec_half_t {
     static constexpr int NE0 = (CMAX + 1) / 2;
     static constexpr int NE1 = CMAX - NE0;
     static constexpr int NEH = NE0 + 1;  // parity edge in the last slot
-    static constexpr int WORDS = (((NEH + 1) / 2) + 3) / 4 * 4;
-    static constexpr int V4 = WORDS / 4;
+    static constexpr int USED = (NEH + 1) / 2;
+    static constexpr int WORDS = USED <= 2 ? USED : (USED + 3) / 4 * 4;
 };
+
+// one half record between global memory and registers, in the widest loads
+// its size allows
+template <int W>
+__device__ __forceinline__ void rec_load(uint32_t* r, const uint32_t* __restrict__ src)
+{
+    if constexpr (W == 1) {
+        r[0] = src[0];
+    } else if constexpr (W == 2) {
+        const uint2 v = *reinterpret_cast<const uint2*>(src);
+        r[0] = v.x;
+        r[1] = v.y;
+    } else {
+#pragma unroll
+        for (int k = 0; k < W / 4; k++) {
+            const uint4 v = reinterpret_cast<const uint4*>(src)[k];
+            r[4 * k] = v.x;
+            r[4 * k + 1] = v.y;
+            r[4 * k + 2] = v.z;
+            r[4 * k + 3] = v.w;
+        }
+    }
+}
+
+template <int W>
+__device__ __forceinline__ void rec_store(uint32_t* __restrict__ dst, const uint32_t* r)
+{
+    if constexpr (W == 1) {
+        dst[0] = r[0];
+    } else if constexpr (W == 2) {
+        *reinterpret_cast<uint2*>(dst) = make_uint2(r[0], r[1]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < W / 4; k++)
+            reinterpret_cast<uint4*>(dst)[k] = make_uint4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
+    }
+}
 
 // One collision group at pair dword tt: APP = sat(...sat(APP + d_1)... + d_n)
 // in slot order.  The group record comes in two 16-byte reads; a record's
@@ -121,6 +161,7 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
     const int tt4 = 4 * tt;
     const int tt4w = tt4 + 4 * HALF;  // byte offset of the pair dword after a wrap
     const bool act = tt < HALF;
+    const int wih = __builtin_amdgcn_readfirstlane(tt >> 6);  // wave index within the half
     const int q = p.q;
     const int quarter = lane * 2 + half;  // which (lane, half) this wave group is
     // the schedule entry of this lane (PAIR: both lanes run lane 0's layers)
@@ -158,7 +199,8 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
     // instead, and every record is written before it is first read (each
     // layer once per iteration).
     const int tl = act ? tt : HALF - 1;
-    auto rec_at = [&](int i, int pair) { return rec_base + (((size_t)i * 2 + half) * HALF + pair) * R::V4; };
+    uint32_t* rec32 = reinterpret_cast<uint32_t*>(rec_base);
+    auto rec_at = [&](int i, int pair) { return rec32 + (((size_t)i * 2 + half) * HALF + pair) * R::WORDS; };
     // Learned parameters: the (beta, alpha) of every (step, lane) entry of an
     // iteration are loaded by thread 2 s + lane during that iteration's
     // syndrome pass and stored in parl before the pass's barrier; a step
@@ -166,8 +208,8 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
     // load nothing.  (Loaded one step ahead in registers instead, the value
     // was copied at the loop's join points, and each copy waited for every
     // load in flight -- the next step's record prefetch included.)
-    static_assert(R::V4 <= 4, "a layer's records fit zero_records");
-    uint4 ra[R::V4], rb[R::V4];
+    static_assert(R::WORDS <= 16, "a layer's records fit zero_records");
+    uint32_t ra[R::WORDS], rb[R::WORDS];
     int si[2], so[2], sc[2], sg0[2];  // layer, lay_off, circulant count, first group of the step in ra / rb
     {
         const int4 e0 = sched_l[sln];
@@ -180,9 +222,9 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
         sc[1] = 0;
         sg0[1] = 0;
 #pragma unroll
-        for (int k = 0; k < R::V4; k++) {
-            ra[k] = make_uint4(0, 0, 0, 0);  // step 0 of iteration 0
-            rb[k] = make_uint4(0, 0, 0, 0);
+        for (int k = 0; k < R::WORDS; k++) {
+            ra[k] = 0;  // step 0 of iteration 0
+            rb[k] = 0;
         }
     }
 
@@ -267,20 +309,15 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
 #pragma unroll
         for (int hs = 0; hs < 2; hs++) {
             const int s = s0 + hs;
-            uint4* cur = hs == 0 ? ra : rb;
-            uint4* nxt = hs == 0 ? rb : ra;
+            uint32_t* cur = hs == 0 ? ra : rb;
+            uint32_t* nxt = hs == 0 ? rb : ra;
             // this step's schedule entry, read one step earlier
             const int i = si[hs];
             const int loff = so[hs];
             const int lcnt = sc[hs];
             uint32_t rw[R::WORDS];
 #pragma unroll
-            for (int k = 0; k < R::V4; k++) {
-                rw[4 * k] = cur[k].x;
-                rw[4 * k + 1] = cur[k].y;
-                rw[4 * k + 2] = cur[k].z;
-                rw[4 * k + 3] = cur[k].w;
-            }
+            for (int k = 0; k < R::WORDS; k++) rw[k] = cur[k];
             {
                 const int sn = s + 1 < p.n_steps ? s + 1 : 0;
                 const int4 en = sched_l[2 * sn + sln];
@@ -290,10 +327,9 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                 sg0[hs ^ 1] = __builtin_amdgcn_readfirstlane(en.w);
                 const int inx = max(si[hs ^ 1], 0);
                 const int itn = s + 1 < p.n_steps ? it : it + 1;  // iteration of the next step
-                const uint4* lay = itn > 0 ? rec_base + ((size_t)inx * 2 + half) * HALF * R::V4 : zero_records;
-                const uint4* src = lay + tl * R::V4;
-#pragma unroll
-                for (int k = 0; k < R::V4; k++) nxt[k] = src[k];
+                const uint32_t* lay = itn > 0 ? rec32 + ((size_t)inx * 2 + half) * HALF * R::WORDS
+                                              : reinterpret_cast<const uint32_t*>(zero_records);
+                rec_load<R::WORDS>(nxt, lay + tl * R::WORDS);
             }
             const bool work = act && !dn && i >= 0;
             const s2 zero = splat(0);
@@ -305,9 +341,11 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
             int ra_[NE0];  // LDS byte address of the edge's pair dword (its read and, outside
                            // the collision slots, its write)
             uint32_t sels[NE0];
-            uint32_t slotm = 0;
-            s2 mn1 = splat(32767), mn2 = splat(32767);
-            uint32_t sg = 0;
+            // set at the start of the read phase and read only under `work`
+            // (no moves for the phi of the idle path)
+            uint32_t slotm;
+            s2 mn1, mn2;
+            uint32_t sg;
             // parity edge: half 0 the own P(i, j), half 1 P(i-1, j) / P(q-1, j-1);
             // check (0, 0) has no second parity edge: its A half reads a +32767
             // placeholder that cannot change min1/min2 or the sign
@@ -319,8 +357,9 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
             // exchange barrier (more groups than wave groups: the rest after
             // it, synchronously).
             const bool mat = prev_def && !dn && prev_g0 + gq < prev_g1;  // wave-uniform
-            int m_base = 0, m_n = 0;
-            uint32_t m_a0 = 0, m_d[6] = {0, 0, 0, 0, 0, 0};
+            // set and read only under mat (no zeroing moves on the common path)
+            int m_base, m_n;
+            uint32_t m_a0, m_d[6];
             if (mat) {
                 const int g = prev_g0 + gq;
                 const int4 ga = *reinterpret_cast<const int4*>(grp_l + GROUP_WORDS * g);
@@ -336,10 +375,17 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                     for (int e = 0; e < 6; e++) m_d[e] = app32[sl[e] + tt];
                 }
             }
-            const bool dummy = half == 1 && i == 0 && tt == 0;
-            s2 xp = zero, mp = zero;
+            // the wave holding thread tt = 0 of half 1 in layer 0 (wave-uniform):
+            // only there does a thread (tt = 0) take the placeholder edge
+            const bool dwave = half == 1 && i == 0 && wih == 0;
+            const bool dummy = dwave && tt == 0;
+            s2 xp, mp;
             const circ_rec* cc = circ + loff + half * n0;
             if (work) {
+                slotm = 0;
+                mn1 = splat(32767);
+                mn2 = splat(32767);
+                sg = 0;
                 // both halves with every slot used (exact degree, even CMAX):
                 // the circulant records and APP dwords of all edges first, so
                 // that the loads are in flight together (the other variants
@@ -388,7 +434,7 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                 {
                     const s2 o = unpack_msg(rw, NEH - 1);
                     uint32_t w = par32[pidx];
-                    if (dummy) w = (w << 16) | 0x7fffu;
+                    if (dwave && dummy) w = (w << 16) | 0x7fffu;
                     xp = __builtin_elementwise_sub_sat(as_s2(w), o);
                     mp = __builtin_elementwise_max(xp, __builtin_elementwise_sub_sat(zero, xp));
                     mn2 = __builtin_elementwise_min(mn2, __builtin_elementwise_max(mn1, mp));
@@ -488,7 +534,9 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                 {
                     s2 yp = out(mp, xp);
                     const s2 np = __builtin_elementwise_add_sat(xp, yp);
-                    if (!dummy) {
+                    if (!dwave) {
+                        par32[pidx] = bits(np);
+                    } else if (!dummy) {
                         par32[pidx] = bits(np);
                     } else {
                         par16[2 * pidx] = np.y;  // low half of dword P(q-1, 179)
@@ -502,10 +550,7 @@ __device__ __forceinline__ int decode_frame_split(unsigned char* lds_raw, const 
                     ow[k] = 2 * k + 1 < NEH ? pack_msg(ys[2 * k], ys[2 * k + 1])
                           : 2 * k < NEH     ? pack_msg(ys[2 * k], zero)
                                             : 0u;
-                uint4* dst = rec_at(i, tt);
-#pragma unroll
-                for (int k = 0; k < R::V4; k++)
-                    dst[k] = make_uint4(ow[4 * k], ow[4 * k + 1], ow[4 * k + 2], ow[4 * k + 3]);
+                rec_store<R::WORDS>(rec_at(i, tt), ow);
             }
             __syncthreads();
             // collision groups of the step: [first group of this step, first
@@ -569,7 +614,8 @@ __global__ void __launch_bounds__(LANES* SPLIT_LANE_THREADS)
     const int cw = blockIdx.x;
     if (active && active[cw] == 0) return;
     const int res = decode_frame_split<CMAX, EXACT, LANES>(
-        lds_raw, llr_all + (size_t)cw * p.N, msg_all + (size_t)cw * p.q * 2 * HALF * rec_half_t<CMAX>::V4,
+        lds_raw, llr_all + (size_t)cw * p.N,
+        reinterpret_cast<uint4*>(reinterpret_cast<uint32_t*>(msg_all) + (size_t)cw * p.q * 2 * HALF * rec_half_t<CMAX>::WORDS),
         hard_all + (size_t)cw * (p.K >> 3), circ, lay_cnt, lay_off, steps, grp_off, step_defer, groups, beta,
         alpha, p);
     if (threadIdx.x == 0) result[cw] = res;
