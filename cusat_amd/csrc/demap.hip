@@ -157,6 +157,10 @@ __device__ __forceinline__ void demap_apsk_symbol(CUSAT_AS_CONST const float* pt
             th0[b] = mn0[b] + margin;
             th1[b] = mn1[b] + margin;
         }
+        // two groups per iteration: the lowest fixed bit is known when the
+        // loop is compiled, and the second group reuses the first's
+        // selections of the bits above it
+#pragma unroll 2
         for (int i0 = 0; i0 < NP; i0 += 8) {
             float tg = low_max;
 #pragma unroll
@@ -192,13 +196,15 @@ __device__ __forceinline__ void demap_apsk_symbol(CUSAT_AS_CONST const float* pt
                         const float m = b < TOP ? mt[b < TOP ? b : 0] : (lpol ? mn1[b] : mn0[b]);
                         const float thr = b < TOP ? tt[b < TOP ? b : 0] : (lpol ? th1[b] : th0[b]);
                         if (__builtin_amdgcn_ballot_w64(d < thr)) {
+                            // A lane with dl == dl0 (every lane whose minimum
+                            // for this bit value is d0) has exp(-dl) == e0
+                            // bit for bit; the exp runs, for every lane, only
+                            // when some lane in range has another dl.
                             const float dl = (d - m) * quarter;
-                            const bool own = m != d0 && dl < 16.0f;
-                            float e = m == d0 ? e0 : 0.0f;
-                            if (__builtin_amdgcn_ballot_w64(own)) {
-                                const float eb = exp_neg(-dl);
-                                e = own ? eb : e;
-                            }
+                            const bool in = dl < 16.0f;
+                            float ev = e0;
+                            if (__builtin_amdgcn_ballot_w64(in && dl != dl0)) ev = exp_neg(-dl);
+                            const float e = in ? ev : 0.0f;
                             if (b < TOP)   at[b < TOP ? b : 0] += e;
                             else if (lpol) s1[b] += e;
                             else           s0[b] += e;
