@@ -237,3 +237,12 @@ def test_unaligned_llr_input(fs, rate, mod, es, offset):
     np.testing.assert_array_equal(hard_a, hard_o)
     np.testing.assert_array_equal(tr.cpu().numpy(), tr_o)
     np.testing.assert_array_equal(hard.cpu().numpy(), hard_o)
+
+
+def test_demap_bench_runs():
+    """The per-code APSK demap benchmark runs on every constellation it
+    lists (two frames, one timed launch each)."""
+    from cusat_amd import demap_bench
+    res = demap_bench.run(frames=2, reps=1, log=lambda s: print(s, flush=True))
+    assert [r["modcod"][2] for r in res] == [m for _, m, _ in demap_bench.APSK]
+    assert all(r["us_per_launch"] > 0 for r in res)
