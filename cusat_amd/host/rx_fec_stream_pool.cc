@@ -67,6 +67,7 @@ rx_fec_stream_pool::rx_fec_stream_pool(fecframe_size_t framesize, const std::vec
             s.n_mod = inf.n_mod;
             s.xfec = inf.xfec_len;
             s.max_dfl = inf.k_bch / 8 - 10;
+            s.q = inf.q;
             std::vector<float> pts(2u << inf.n_mod);
             uint32_t cols[8];
             int nm = 0, rows = 0;
@@ -81,7 +82,7 @@ rx_fec_stream_pool::rx_fec_stream_pool(fecframe_size_t framesize, const std::vec
         }
         const size_t mb = (size_t)micro_batch;
         const size_t state_bytes = std::max<size_t>(d_mc.size(), 1) * 4 * sizeof(float);
-        d_res_bytes = round256(3 * mb * sizeof(int32_t));
+        d_res_bytes = round256(4 * mb * sizeof(int32_t));  // trials | valid | dfl | fallback
         d_state_bytes = round256(state_bytes);
         hip_check(hipMalloc(&d_state, state_bytes), "state");
         {
@@ -96,7 +97,10 @@ rx_fec_stream_pool::rx_fec_stream_pool(fecframe_size_t framesize, const std::vec
             hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
             r.stream = st;
             if (d_ctx.empty()) continue;
-            abi_check(cusat_stream_create(&r.sc, d_ctx.data(), (int)d_ctx.size(), micro_batch, 0), "stream context");
+            // learned mode: a MODCOD decodes learned only once set_learned_params
+            // gave its context parameters; until then it decodes plain
+            abi_check(cusat_stream_create(&r.sc, d_ctx.data(), (int)d_ctx.size(), micro_batch, CUSAT_STREAM_LEARNED),
+                      "stream context");
             hip_check(hipMalloc(&r.sym_dev, mb * (d_max_xfec + 8) * sizeof(complexf)), "device buffers");
             hip_check(hipMalloc(&r.blk, d_res_bytes + d_state_bytes + std::max<size_t>(mb * d_max_dfl, 1)),
                       "device buffers");
@@ -108,7 +112,13 @@ rx_fec_stream_pool::rx_fec_stream_pool(fecframe_size_t framesize, const std::vec
             abi_check(cusat_stream_rx_plan_async(r.sc, nullptr, 0, nullptr, 0, st), "plan");
             int32_t* res = r.res_dev;
             const float* sym = reinterpret_cast<const float*>(r.sym_dev);
-            auto rx = [&] { return cusat_stream_rx(r.sc, sym, r.out_dev, res, res + mb, res + 2 * mb, st); };
+            // cusat_stream_rx's three launches, with the decode's fallback flags kept
+            auto rx = [&] {
+                int rc = cusat_stream_rx_demap(r.sc, sym, st);
+                if (rc == CUSAT_OK) rc = cusat_stream_rx_decode(r.sc, res, res + mb, res + 3 * mb, st);
+                if (rc == CUSAT_OK) rc = cusat_stream_rx_bb(r.sc, r.out_dev, res, res + mb, res + 2 * mb, st);
+                return rc;
+            };
             auto est = [&] { return cusat_stream_rx_dd_estimate(r.sc, sym, st); };
             auto resolve = [&] { return cusat_stream_rx_dd_resolve(r.sc, d_state, r.state_dev, st); };
             abi_check(est(), "snr");
@@ -391,6 +401,7 @@ void rx_fec_stream_pool::post_process(flight_slot& s, block*& cur)
     const int32_t* trials = s.res;
     const int32_t* valid = s.res + mb;
     const int32_t* dfl = s.res + 2 * mb;
+    const int32_t* fallback = s.res + 3 * mb;
     if (d_gpu_dd)
         for (auto& m : d_mc) {
             // the state after this micro-batch, unless an update_snr came later
@@ -403,6 +414,7 @@ void rx_fec_stream_pool::post_process(flight_slot& s, block*& cur)
     for (int i = 0; i < s.n_frames; i++) {
         modcod_state& m = d_mc[s.mc[i]];
         m.frames++;
+        if (fallback[i] == 1) m.fallbacks++;
         if (!rx_fec_chain::frame_passes_fec_checks(trials[i], valid[i] == 1)) {
             m.errors++;
             continue;
@@ -505,7 +517,7 @@ void rx_fec_stream_pool::reset_all()
         m.snr_db = 0;
         m.noise_var = 1.0f;
         m.llr_gain = 4.0f;
-        m.frames = m.errors = 0;
+        m.frames = m.errors = m.fallbacks = 0;
     }
     reset_state();  // every launch has completed
 }
@@ -534,6 +546,32 @@ uint64_t rx_fec_stream_pool::total_error_count() const
 {
     uint64_t n = 0;
     for (const auto& m : d_mc) n += m.errors;
+    return n;
+}
+
+bool rx_fec_stream_pool::set_learned_params(uint8_t modcod, const int16_t* beta, const int16_t* alpha, int n_iters)
+{
+    const auto it = d_by_idx.find(modcod);
+    if (it == d_by_idx.end()) return false;
+    // micro-batches in flight may still read the parameter buffer that the
+    // call rewrites in place; the next plan uploads the new descriptors
+    for (auto& r : d_streams) hip_check(hipStreamSynchronize(r.stream), "sync");
+    abi_check(cusat_ldpc_set_params(d_ctx[it->second], n_iters > 0 ? beta : nullptr, n_iters > 0 ? alpha : nullptr,
+                                    n_iters),
+              "learned parameters");
+    return true;
+}
+
+int rx_fec_stream_pool::layer_count(uint8_t modcod) const
+{
+    const auto it = d_by_idx.find(modcod);
+    return it == d_by_idx.end() ? -1 : d_mc[it->second].q;
+}
+
+uint64_t rx_fec_stream_pool::learned_fallback_count() const
+{
+    uint64_t n = 0;
+    for (const auto& m : d_mc) n += m.fallbacks;
     return n;
 }
 

@@ -38,6 +38,10 @@
  *    gpu_decode_budget does; frames are dropped (and counted) only when the
  *    ring is full: max(8, 512 / micro_batch) micro-batches, about the 512
  *    frames of a chain's 128 micro-batches of 4;
+ *  - set_learned_params gives a MODCOD the learned offset / normalised
+ *    min-sum with in-launch plain fallback (cusat_stream_rx in
+ *    CUSAT_STREAM_LEARNED mode); the equality with rx_fec_pool above holds
+ *    for the MODCODs without learned parameters;
  *  - HIP errors throw std::runtime_error.
  */
 #ifndef CUSAT_AMD_DSP_RX_FEC_STREAM_POOL_H
@@ -110,6 +114,22 @@ public:
      *  stream of 6144 frames: 128-151 k vs 112-126 k frames/s).  Set before
      *  the first submit. */
     void set_gpu_dd_snr(bool on) { d_gpu_dd = on; }
+    /** Learned offset / normalised min-sum for one enabled MODCOD (no
+     *  counterpart in rx_fec_pool): beta and alpha are int16 [n_iters][q]
+     *  tables (q = layer_count(modcod)) of offsets in LLR units and scales in
+     *  64ths (0 = no scale), as cusat_ldpc_set_params takes them.  Frames of
+     *  that MODCOD launched afterwards decode n_iters learned iterations, and
+     *  any frame failing frame_passes_fec_checks is decoded again by the plain
+     *  decoder (max_trials) in the same launch; n_iters = 0 goes back to
+     *  plain.  Waits for the micro-batches in flight.  Returns false for a
+     *  MODCOD that is not enabled; bad tables throw std::runtime_error. */
+    bool set_learned_params(uint8_t modcod, const int16_t* beta, const int16_t* alpha, int n_iters);
+    /** Layers (check-row groups of 360) of an enabled MODCOD's code: the row
+     *  length of the learned tables; -1 when not enabled. */
+    int layer_count(uint8_t modcod) const;
+    /** Frames whose learned pass was redone by the plain decoder (not in
+     *  rx_fec_pool: it has no learned decoder). */
+    uint64_t learned_fallback_count() const;
     /** Wait for the oldest micro-batch instead of dropping a frame when the
      *  ring is full (rx_fec_chain::set_block_when_full). */
     void set_block_when_full(bool on) { d_block_when_full = on; }
@@ -123,12 +143,12 @@ private:
         uint8_t idx = 0;
         int code = 0;  // context index in the stream contexts
         modulation_t modulation = MOD_QPSK;
-        int n_mod = 0, xfec = 0, max_dfl = 0;
+        int n_mod = 0, xfec = 0, max_dfl = 0, q = 0;
         std::vector<complexf> points;
         float snr_db = 0, pilot_snr_db = 0, noise_var = 1.0f, llr_gain = 4.0f;
         int snr_skip = 0;
         int host_event_batch = -1;  // micro-batch that carries the latest update_snr
-        uint64_t frames = 0, errors = 0;
+        uint64_t frames = 0, errors = 0, fallbacks = 0;
     };
     struct flight_slot {
         complexf* syms = nullptr;  // pinned, frames at 64-byte aligned offsets

@@ -11,7 +11,11 @@
 //    collects it; reset_all drops queued output; warmup_all leaves clean
 //    counters; a full ring drops frames (counted as errors) while the rest
 //    still leave in arrival order, or blocks when set_block_when_full is on;
-//    an unknown MODCOD index in the constructor throws.
+//    an unknown MODCOD index in the constructor throws;
+//  - learned parameters per MODCOD (set_learned_params): tables that restate
+//    the plain check node give the plain pool's output, tables that stop
+//    every learned pass send each frame to the in-launch plain fallback
+//    (learned_fallback_count) and still give it.
 // Fewer than 32 frames of each MODCOD are sent, so the decision-directed SNR
 // re-estimate (every 32nd frame of a MODCOD) never fires and both pools scale
 // every frame by the update_snr value.
@@ -299,6 +303,58 @@ int main()
                  int want_kept = 0;  // non-empty payloads among the frames that were kept
                  for (int k = 0; k < (block ? n : 510); k++) want_kept += t.payloads[sent[k]].empty() ? 0 : 1;
                  QA_CHECK_EQ(kept, want_kept);
+             }
+         }},
+        {"learned_parameters",
+         [] {
+             // learned tables for MODCODs 4 and 14 that restate the plain check
+             // node (offset 1, no scale): every output byte, size and counter
+             // equals the plain pool's, the fallback frames being redone by
+             // the plain decoder in the same launch; tables that silence the
+             // messages (offset 127, scale 1/64): with thousands of channel
+             // bit errors per frame no learned pass converges, every decoded
+             // frame of 4 and 14 falls back, the output is still the plain
+             // pool's
+             const std::vector<source> src = make_sources();
+             const std::vector<uint8_t> ids = {4, 14, 18, 26};
+             const std::vector<arrival> arr = make_arrivals(src, 5);
+             rx_fec_stream_pool plain(FECFRAME_NORMAL, ids, 50, 64, 3, 4);
+             pool_output want;
+             feed(plain, src, arr, want);
+             QA_CHECK_EQ(plain.learned_fallback_count(), (uint64_t)0);
+             uint64_t decoded = 0;  // frames of MODCODs 4 and 14 of the right length
+             for (const auto& a : arr) decoded += (a.frame >= 0 && (a.src == 0 || a.src == 1)) ? 1 : 0;
+             for (int kind = 0; kind < 2; kind++) {
+                 rx_fec_stream_pool pool(FECFRAME_NORMAL, ids, 50, 64, 3, 4);
+                 QA_CHECK_EQ(pool.layer_count(7), -1);
+                 QA_CHECK(!pool.set_learned_params(7, nullptr, nullptr, 0));
+                 const int T = 20;
+                 for (uint8_t id : {(uint8_t)4, (uint8_t)14}) {
+                     const int q = pool.layer_count(id);
+                     QA_CHECK(q > 0);
+                     std::vector<int16_t> beta((size_t)T * q, (int16_t)(kind == 0 ? 1 : 127));
+                     std::vector<int16_t> alpha((size_t)T * q, (int16_t)(kind == 0 ? 0 : 1));
+                     QA_CHECK(pool.set_learned_params(id, beta.data(), alpha.data(), T));
+                 }
+                 pool_output got;
+                 feed(pool, src, arr, got);
+                 QA_CHECK(got.bytes == want.bytes);
+                 QA_CHECK(got.sizes == want.sizes);
+                 QA_CHECK_EQ(pool.total_frame_count(), plain.total_frame_count());
+                 QA_CHECK_EQ(pool.total_error_count(), plain.total_error_count());
+                 const uint64_t fb = pool.learned_fallback_count();
+                 std::printf("learned tables %s: %llu of %llu frames of MODCODs 4 and 14 fell back\n",
+                             kind == 0 ? "offset 1" : "offset 127, scale 1/64", (unsigned long long)fb,
+                             (unsigned long long)decoded);
+                 if (kind == 0) QA_CHECK(fb < decoded);
+                 else QA_CHECK_EQ(fb, decoded);
+                 // back to plain: no more fallbacks
+                 pool.reset_all();
+                 for (uint8_t id : {(uint8_t)4, (uint8_t)14}) QA_CHECK(pool.set_learned_params(id, nullptr, nullptr, 0));
+                 pool_output again;
+                 feed(pool, src, arr, again);
+                 QA_CHECK(again.bytes == want.bytes);
+                 QA_CHECK_EQ(pool.learned_fallback_count(), (uint64_t)0);
              }
          }},
         {"unknown_index_in_constructor_throws",
